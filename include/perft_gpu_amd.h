@@ -1,0 +1,144 @@
+/*
+ * perft_gpu_amd.h -- C ABI of the MI355X perft engine.
+ *
+ * Plain pointers and sizes only; no C++ or torch types cross this boundary.
+ * Each entry point names the reference interface it replaces (file:line in
+ * sarona-RD/perft_gpu).
+ *
+ * Conventions (the reference printf()s and exit()s on errors and signals OOM
+ * in-band with perfts[0] == ~0ull, perft_bb.h:1352-1363; here):
+ *   - every call returns a pf_status; the library never prints or exits;
+ *   - host arrays are caller-owned; the device arena, tables and the
+ *     transposition table are library-owned;
+ *   - one pf_ctx per GPU; calls on one context are serialised internally,
+ *     pf_submit / pf_poll may be called from any thread while the serving loop
+ *     runs.
+ */
+#ifndef PERFT_GPU_AMD_H
+#define PERFT_GPU_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Position, byte-for-byte the reference HexaBitBoardPosition (chess.h:208-230):
+ * six bitboards (a1 = bit 0); pawns bits 0-1 white castle (1 = O-O, 2 = O-O-O),
+ * 2-3 black castle, 4-7 en-passant file + 1, 56-62 half-move counter (ignored),
+ * 63 side to move (1 = black). */
+typedef struct pf_pos48 {
+    uint64_t white_pieces;
+    uint64_t pawns;
+    uint64_t knights;
+    uint64_t bishop_queens;
+    uint64_t rook_queens;
+    uint64_t kings;
+} pf_pos48;
+
+typedef enum pf_status {
+    PF_OK = 0,
+    PF_EINVAL = -1,   /* bad argument, illegal FEN, depth out of range */
+    PF_ENOMEM = -2,   /* device memory exhausted (reference: ALLSET sentinel) */
+    PF_EDEVICE = -3,  /* HIP error or no usable GPU */
+    PF_EBUSY = -4,    /* serving queue full */
+    PF_EAGAIN = -5    /* nothing to report yet */
+} pf_status;
+
+typedef enum pf_backend {
+    PF_BACKEND_KS = 0,     /* Kogge-Stone sliders (USE_SLIDING_LUT 0) */
+    PF_BACKEND_PLAIN = 1,  /* plain magics, 2.25 MiB (USE_FANCY_MAGICS 0) */
+    PF_BACKEND_FANCY = 2,  /* fixed-shift fancy magics, 760 KiB (reference default) */
+    PF_BACKEND_BYTE = 3    /* byte-lookup fancy magics, 146 KiB, staged in LDS */
+} pf_backend;
+
+typedef enum pf_mode {
+    PF_MODE_AUTO = 0,        /* persistent single launch when it fits, else levels */
+    PF_MODE_LEVELS = 1,      /* host-driven level loop with depth-first chunking */
+    PF_MODE_PERSISTENT = 2   /* one cooperative launch for the whole batch */
+} pf_mode;
+
+typedef struct pf_config {
+    int backend;              /* pf_backend */
+    int device;               /* HIP device ordinal */
+    uint64_t arena_bytes;     /* frontier arena; 0 = 60% of free HBM after the TT */
+    uint64_t tt_bytes;        /* transposition table; 0 = off */
+    uint64_t zobrist_seed;    /* seed of the xorshift64 Zobrist tables; 0 = default */
+    const char *tune_file;    /* autotuner result file to load; NULL = built-in */
+} pf_config;
+
+typedef struct pf_ctx pf_ctx;
+
+typedef struct pf_result {
+    uint64_t tag;        /* caller's tag from pf_submit */
+    uint64_t nodes;      /* perft count */
+    int32_t status;      /* pf_status */
+    int32_t depth;
+    double submit_s;     /* arrival time (library clock, seconds) */
+    double done_s;       /* completion time */
+} pf_result;
+
+typedef struct pf_stats {
+    uint64_t kernel_launches;
+    uint64_t levels_run;
+    uint64_t chunks;          /* depth-first chunk splits taken */
+    uint64_t peak_arena_bytes;
+    uint64_t tt_probes, tt_hits, tt_stores;
+    double gpu_ms;            /* device time of expand kernels (HIP events) */
+    double last_expand_ms;    /* average duration of the last run's expand launches */
+    uint64_t last_expand_launches;
+} pf_stats;
+
+/* Replaces initGPU + MoveGeneratorBitboard::init + setupHashTables128b
+ * (launcher.h:1506-1533, MoveGeneratorBitboard.h:982-1203, launcher.h:293-320). */
+pf_status pf_init(const pf_config *cfg, pf_ctx **out);
+/* Replaces MoveGeneratorBitboard::destroy / freeCompleteTT (MoveGeneratorBitboard.h:1205). */
+void pf_destroy(pf_ctx *ctx);
+const char *pf_strerror(int status);
+
+/* Replaces Utils::readFENString + Utils::board088ToHexBB (util.cpp:189-280, 92-140):
+ * same quirks -- the en-passant field keeps only its file, clocks are ignored. */
+pf_status pf_fen_to_pos(const char *fen, pf_pos48 *out);
+pf_status pf_pos_to_fen(const pf_pos48 *pos, char *buf, int buflen);
+
+/* perft(position, depth): replaces perftLauncher (launcher.h:1455-1504), which
+ * prints instead of returning.  depth 0 -> 1.  Plain per-position path: a
+ * host-driven level loop, no cross-position batching. */
+pf_status pf_perft(pf_ctx *ctx, const pf_pos48 *pos, int depth, uint64_t *out);
+
+/* Batched perft of n positions (any mix of sides to move): replaces
+ * perft_bb_gpu_simple_hash(count, positions, ..., perfts, depth, ...)
+ * (perft_bb.h:1242), which needs one side to move and <= 256 roots. */
+pf_status pf_perft_batch(pf_ctx *ctx, const pf_pos48 *pos, int n, int depth, uint64_t *out);
+pf_status pf_perft_batch_mode(pf_ctx *ctx, const pf_pos48 *pos, int n, int depth, int mode, uint64_t *out);
+
+/* Divided perft for the UCI "go perft": per legal root move, the move
+ * (from | to << 6 | flag << 12, reference CMove flags without the capture
+ * bit) and its perft(depth - 1).  moves/counts hold >= 256 entries. */
+pf_status pf_divide(pf_ctx *ctx, const pf_pos48 *pos, int depth, uint16_t *moves, uint64_t *counts, int *n);
+/* UCI text of a move in that encoding, e.g. "e7e8q"; buf holds >= 6 bytes. */
+pf_status pf_move_to_uci(uint16_t move, char *buf);
+/* Apply a UCI move text to a position (for "position ... moves ..."). */
+pf_status pf_apply_uci(pf_pos48 *pos, const char *uci);
+
+/* Slider backend selection at run time (the reference's switches.h:102-125
+ * are compile-time). */
+pf_status pf_set_backend(pf_ctx *ctx, int backend);
+pf_status pf_get_stats(pf_ctx *ctx, pf_stats *out);
+pf_status pf_reset_stats(pf_ctx *ctx);
+
+/* Seeded random-playout positions (workload generator, no reference
+ * counterpart): xorshift64(13,7,17) stream; per candidate, a length uniform in
+ * [min_plies, max_plies], each ply a uniform pick from the legal moves in
+ * ascending (from, to, promotion N<B<R<Q) order; candidates ending in mate or
+ * stalemate, with fewer than min_moves legal moves, or (no_check) in check are
+ * discarded.  Returns PF_OK when n positions were produced. */
+pf_status pf_playouts(uint64_t seed, int n, int min_plies, int max_plies, int min_moves, int no_check,
+                      pf_pos48 *out);
+/* Number of legal moves of a position (host generator; for tests and tools). */
+int pf_count_moves_host(const pf_pos48 *pos);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,254 @@
+"""perft_gpu_amd -- Python host mirror of the MI355X perft engine.
+
+Thin ctypes layer over the C ABI in include/perft_gpu_amd.h (library
+perft_gpu_amd/_build/libperft_gpu_amd.so, built by perft_gpu_amd.build).
+There is no CPU fallback: if the library is missing, or no GPU is usable,
+every compute call raises.
+
+Reference interface mirrored (sarona-RD/perft_gpu):
+  perft_launcher(pos, depth)        perftLauncher        (launcher.h:1455-1504)
+  Engine.perft_batch(positions, d)  perft_bb_gpu_simple_hash(count, ...) (perft_bb.h:1242)
+  fen_to_pos(fen)                   Utils::readFENString + board088ToHexBB (util.cpp:189, :92)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libperft_gpu_amd.so")
+CLI_PATH = os.path.join(HERE, "_build", "perft_gpu")
+
+PF_OK, PF_EINVAL, PF_ENOMEM, PF_EDEVICE, PF_EBUSY, PF_EAGAIN = 0, -1, -2, -3, -4, -5
+BACKENDS = {"ks": 0, "plain": 1, "fancy": 2, "byte": 3}
+MODE_AUTO, MODE_LEVELS, MODE_PERSISTENT = 0, 1, 2
+START_FEN = "rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1"
+
+# structured dtype with the exact pf_pos48 layout, for zero-copy batches
+POS_DTYPE = np.dtype([("white_pieces", "<u8"), ("pawns", "<u8"), ("knights", "<u8"),
+                      ("bishop_queens", "<u8"), ("rook_queens", "<u8"), ("kings", "<u8")])
+
+
+class PerftError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        super().__init__("%s: status %d (%s)" % (what or "perft_gpu_amd", status, _strerror(status)))
+
+
+class Pos48(ctypes.Structure):
+    _fields_ = [("white_pieces", ctypes.c_uint64), ("pawns", ctypes.c_uint64), ("knights", ctypes.c_uint64),
+                ("bishop_queens", ctypes.c_uint64), ("rook_queens", ctypes.c_uint64), ("kings", ctypes.c_uint64)]
+
+    def as_tuple(self):
+        return (self.white_pieces, self.pawns, self.knights, self.bishop_queens, self.rook_queens, self.kings)
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("backend", ctypes.c_int), ("device", ctypes.c_int), ("arena_bytes", ctypes.c_uint64),
+                ("tt_bytes", ctypes.c_uint64), ("zobrist_seed", ctypes.c_uint64), ("tune_file", ctypes.c_char_p)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("nodes", ctypes.c_uint64), ("status", ctypes.c_int32),
+                ("depth", ctypes.c_int32), ("submit_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("kernel_launches", ctypes.c_uint64), ("levels_run", ctypes.c_uint64), ("chunks", ctypes.c_uint64),
+                ("peak_arena_bytes", ctypes.c_uint64), ("tt_probes", ctypes.c_uint64), ("tt_hits", ctypes.c_uint64),
+                ("tt_stores", ctypes.c_uint64), ("gpu_ms", ctypes.c_double), ("last_expand_ms", ctypes.c_double),
+                ("last_expand_launches", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+# every symbol the C ABI declares (checked by the CPU test suite)
+EXPORTS = ["pf_init", "pf_destroy", "pf_strerror", "pf_fen_to_pos", "pf_pos_to_fen", "pf_perft",
+           "pf_perft_batch", "pf_perft_batch_mode", "pf_divide", "pf_move_to_uci", "pf_apply_uci",
+           "pf_set_backend", "pf_get_stats", "pf_reset_stats", "pf_playouts", "pf_count_moves_host"]
+
+
+def load_library(path=LIB_PATH):
+    """Load the engine library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("perft_gpu_amd library not built (%s); run `python -m perft_gpu_amd.build`" % path)
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    u64p = P(ctypes.c_uint64)
+    sig = {
+        "pf_init": (ctypes.c_int, [P(Config), P(ctypes.c_void_p)]),
+        "pf_destroy": (None, [ctypes.c_void_p]),
+        "pf_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+        "pf_fen_to_pos": (ctypes.c_int, [ctypes.c_char_p, P(Pos48)]),
+        "pf_pos_to_fen": (ctypes.c_int, [P(Pos48), ctypes.c_char_p, ctypes.c_int]),
+        "pf_perft": (ctypes.c_int, [ctypes.c_void_p, P(Pos48), ctypes.c_int, u64p]),
+        "pf_perft_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, u64p]),
+        "pf_perft_batch_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, u64p]),
+        "pf_divide": (ctypes.c_int, [ctypes.c_void_p, P(Pos48), ctypes.c_int, P(ctypes.c_uint16), u64p,
+                                     P(ctypes.c_int)]),
+        "pf_move_to_uci": (ctypes.c_int, [ctypes.c_uint16, ctypes.c_char_p]),
+        "pf_apply_uci": (ctypes.c_int, [P(Pos48), ctypes.c_char_p]),
+        "pf_set_backend": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "pf_get_stats": (ctypes.c_int, [ctypes.c_void_p, P(Stats)]),
+        "pf_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
+        "pf_playouts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_void_p]),
+        "pf_count_moves_host": (ctypes.c_int, [P(Pos48)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    for name, (res, args) in _optional_sigs().items():
+        if hasattr(lib, name):
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _optional_sigs():
+    P = ctypes.POINTER
+    return {
+        "pf_submit": (ctypes.c_int, [ctypes.c_void_p, P(Pos48), ctypes.c_int, ctypes.c_uint64]),
+        "pf_poll": (ctypes.c_int, [ctypes.c_void_p, P(Result), ctypes.c_int]),
+        "pf_serve_start": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64]),
+        "pf_serve_stop": (ctypes.c_int, [ctypes.c_void_p]),
+        "pf_now": (ctypes.c_double, []),
+        "pf_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
+    }
+
+
+def _strerror(status):
+    try:
+        return load_library().pf_strerror(status).decode()
+    except Exception:
+        return "?"
+
+
+def _check(status, what):
+    if status != PF_OK:
+        raise PerftError(status, what)
+
+
+def fen_to_pos(fen):
+    p = Pos48()
+    _check(load_library().pf_fen_to_pos(fen.encode(), ctypes.byref(p)), "fen_to_pos(%r)" % fen)
+    return p
+
+
+def pos_to_fen(pos):
+    buf = ctypes.create_string_buffer(128)
+    _check(load_library().pf_pos_to_fen(ctypes.byref(pos), buf, 128), "pos_to_fen")
+    return buf.value.decode()
+
+
+def move_to_uci(move):
+    buf = ctypes.create_string_buffer(8)
+    load_library().pf_move_to_uci(move, buf)
+    return buf.value.decode()
+
+
+def apply_uci(pos, uci):
+    p = Pos48(*pos.as_tuple())
+    _check(load_library().pf_apply_uci(ctypes.byref(p), uci.encode()), "apply_uci(%r)" % uci)
+    return p
+
+
+def positions_array(positions):
+    """list of Pos48 / structured array -> contiguous POS_DTYPE array"""
+    if isinstance(positions, np.ndarray) and positions.dtype == POS_DTYPE:
+        return np.ascontiguousarray(positions)
+    arr = np.zeros(len(positions), dtype=POS_DTYPE)
+    for i, p in enumerate(positions):
+        arr[i] = p.as_tuple() if hasattr(p, "as_tuple") else tuple(int(x) for x in p)
+    return arr
+
+
+def playouts(seed, n, min_plies, max_plies, min_moves=0, no_check=False):
+    arr = np.zeros(n, dtype=POS_DTYPE)
+    _check(load_library().pf_playouts(seed, n, min_plies, max_plies, min_moves, int(bool(no_check)),
+                                      arr.ctypes.data_as(ctypes.c_void_p)), "playouts")
+    return arr
+
+
+class Engine:
+    """One device context (pf_ctx).  Mirrors the reference's GPU perft entry
+    points; every compute call runs the HIP kernels."""
+
+    def __init__(self, backend="ks", device=0, arena_bytes=0, tt_bytes=0, zobrist_seed=0, tune_file=None):
+        lib = load_library()
+        self._lib = lib
+        cfg = Config(BACKENDS[backend] if isinstance(backend, str) else int(backend), device, arena_bytes,
+                     tt_bytes, zobrist_seed, tune_file.encode() if tune_file else None)
+        ctx = ctypes.c_void_p()
+        _check(lib.pf_init(ctypes.byref(cfg), ctypes.byref(ctx)), "pf_init")
+        self._ctx = ctx
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.pf_destroy(self._ctx)
+            self._ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_backend(self, backend):
+        _check(self._lib.pf_set_backend(self._ctx, BACKENDS[backend] if isinstance(backend, str) else int(backend)),
+               "set_backend")
+
+    def perft(self, pos, depth):
+        out = ctypes.c_uint64()
+        _check(self._lib.pf_perft(self._ctx, ctypes.byref(pos), depth, ctypes.byref(out)), "perft")
+        return out.value
+
+    def perft_batch(self, positions, depth, mode=MODE_AUTO):
+        arr = positions_array(positions)
+        out = np.zeros(len(arr), dtype=np.uint64)
+        _check(self._lib.pf_perft_batch_mode(self._ctx, arr.ctypes.data_as(ctypes.c_void_p), len(arr), depth, mode,
+                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))), "perft_batch")
+        return out
+
+    def divide(self, pos, depth):
+        mv = (ctypes.c_uint16 * 256)()
+        cnt = (ctypes.c_uint64 * 256)()
+        n = ctypes.c_int()
+        _check(self._lib.pf_divide(self._ctx, ctypes.byref(pos), depth, mv, cnt, ctypes.byref(n)), "divide")
+        return [(move_to_uci(mv[i]), cnt[i]) for i in range(n.value)]
+
+    def stats(self):
+        s = Stats()
+        _check(self._lib.pf_get_stats(self._ctx, ctypes.byref(s)), "stats")
+        return s.as_dict()
+
+    def reset_stats(self):
+        _check(self._lib.pf_reset_stats(self._ctx), "reset_stats")
+
+
+def perft_launcher(pos, depth, engine=None):
+    """perftLauncher(pos, depth, launchDepth) (launcher.h:1455): returns the
+    count instead of printing it; launch depth is chosen by the engine."""
+    own = engine is None
+    eng = engine or Engine()
+    try:
+        return eng.perft(pos, depth)
+    finally:
+        if own:
+            eng.close()

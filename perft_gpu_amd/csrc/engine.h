@@ -1,0 +1,62 @@
+// engine.h -- host runtime of the perft engine: device context, frontier
+// arena, level driver with depth-first chunking.
+#pragma once
+#include <mutex>
+#include <vector>
+
+#include "../../include/perft_gpu_amd.h"
+#include "convert.h"
+#include "kernels.h"
+
+namespace pf {
+
+struct Engine {
+    std::recursive_mutex mu;
+    int device = 0;
+    int backend = kKS;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    SliderTables* d_tab = nullptr;
+
+    // frontier arena (stack allocator)
+    char* arena = nullptr;
+    u64 arena_bytes = 0, arena_top = 0;
+
+    // control block: [0] err, [1] tile counter, [2] out counter
+    u32* d_ctrl = nullptr;
+    unsigned long long* d_total = nullptr;   // range_count output
+    unsigned long long* d_sums = nullptr;    // final per-root sums
+    unsigned long long* d_wsums = nullptr;   // per-root sums of expand-and-write launches
+    u64 sums_cap = 0;
+    unsigned long long* h_pinned = nullptr;  // small pinned readback buffer
+
+    int grid[4][2] = {};
+    int pgrid[4] = {};
+    PersistCtl* d_pctl = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    pf_stats st = {};
+    double run_expand_ms = 0;
+    u64 run_expand_launches = 0;
+    bool timing = true;
+
+    int depth = 0;  // depth of the current run
+
+    pf_status init(const pf_config& cfg);
+    void destroy();
+    pf_status ensure_sums(u64 n);
+    pf_status perft_levels(const pf_pos48* pos, int n, int depth, u64* out);
+    pf_status perft_persistent(const pf_pos48* pos, int n, int depth, u64* out);
+    pf_status init_tt(u64 bytes, u64 seed);
+    void load_tuning(const char* path);
+
+  private:
+    char* alloc(u64 bytes);
+    void release(char* p);
+    pf_status expand(int mode, const Board* in, const u32* in_root, u64 n_in, Board* out, u32* out_root,
+                     u64 out_cap, unsigned long long* sums, u64* next_total);
+    pf_status range_count(const Board* b, u64 n, u64* total);
+    pf_status upload_roots(const pf_pos48* pos, int n, Board** roots, u32** rid);
+    pf_status descend(const Board* lvl, const u32* rts, u64 n, u64 n_next, int L);
+};
+
+}  // namespace pf

@@ -1,0 +1,62 @@
+// kernels.h -- host-callable launchers for the HIP kernels in kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "movegen.h"
+
+namespace pf {
+
+// Device view of the slider tables (pointers into one device blob).
+struct DevTables {
+    const SliderTables* t;
+};
+
+// One breadth-first level: parents `in[0..n_in)` with their root ids.
+struct LevelArgs {
+    const Board* in;
+    const u32* in_root;
+    u32 n_in;
+    Board* out;          // children (expand-and-write mode only)
+    u32* out_root;
+    u64 out_cap;         // capacity of out / out_root in nodes
+    u32* out_count;      // device bump counter for out
+    unsigned long long* root_sums;  // per-root accumulators (nullptr: skip)
+    unsigned long long* total;      // sum over all children (nullptr: skip)
+    u32* tile_ctr;       // persistent tile queue counter (zero before launch)
+    u32* err;            // set non-zero on capacity overflow
+};
+
+enum ExpandMode : int { kLeaf = 0, kWrite = 1 };
+
+// Occupancy-derived grid size for the expand kernel (workgroups).
+int expand_grid(int backend, int mode, int n_cu);
+
+hipError_t launch_init_roots(int backend, const DevTables& dt, Board* roots, u32 n,
+                             unsigned long long* root_sums, hipStream_t s);
+hipError_t launch_expand(int backend, int mode, const DevTables& dt, const LevelArgs& a,
+                         int grid, hipStream_t s);
+hipError_t launch_range_count(const Board* b, u32 n, unsigned long long* out, hipStream_t s);
+
+// Whole-batch breadth-first perft in one cooperative launch (grid barrier
+// between levels).  Levels 1..depth-2 are bump-allocated in `arena`.
+struct PersistCtl {
+    u32 tile_ctr[20];
+    u32 out_count[20];
+    u32 err;
+    u32 pad[3];
+    unsigned long long totals[20];  // totals[L] = number of nodes of level L
+};
+struct PersistArgs {
+    Board* roots;
+    u32* root_ids;
+    u32 n_roots;
+    int depth;
+    char* arena;
+    u64 arena_bytes;
+    PersistCtl* ctl;
+    unsigned long long* sums;
+};
+int persistent_grid(int backend, int n_cu);
+hipError_t launch_persistent(int backend, const DevTables& dt, const PersistArgs& a, int grid, hipStream_t s);
+
+}  // namespace pf

@@ -1,0 +1,349 @@
+// movegen.h -- legal move counting, generation and make-move on the
+// side-to-move-is-us Board, for host and device.
+//
+// Semantics follow the reference move generator exactly, including its
+// conventions on unreachable positions (MoveGeneratorBitboard.h):
+//   * pins: an enemy slider on an empty-board king ray with exactly one piece
+//     between pins that piece (findPinnedPieces, :1213-1249);
+//   * threatened squares are computed with our king removed (x-ray,
+//     findAttackedSquares, :1254-1298);
+//   * en passant: a pinned capturer may capture only along its pin line; an
+//     unpinned one is rejected only by the horizontal discovered-check test
+//     (:3071-3106); in check, en passant counts only if the captured pawn is
+//     the single checker (:2873-2890);
+//   * in check, pinned pieces never move (:2867) and a pawn double push is
+//     tried only if the single push does not resolve the check (:2906-2920);
+//   * castling needs its right, empty F1G1 / B1D1 and unattacked F1G1 / C1D1
+//     (:3180-3214);
+//   * promotions count four (:3160-3178).
+// The counting itself is organised for wave64 SIMT: pawns, knights and (with
+// Kogge-Stone) sliders are counted set-wise with no per-piece loops.
+#pragma once
+#include "sliders.h"
+
+namespace pf {
+
+// Move encoding (16 bits): from | to << 6 | flag << 12.  Flags follow the
+// reference CMove flags without the capture bit (chess.h:259-293).
+enum : int {
+    kFlQuiet = 0, kFlDouble = 1, kFlOO = 2, kFlOOO = 3, kFlEP = 5,
+    kFlPromoN = 8, kFlPromoB = 9, kFlPromoR = 10, kFlPromoQ = 11
+};
+PF_HD u32 mk_move(int from, int to, int flag) { return (u32)(from | (to << 6) | (flag << 12)); }
+
+struct PosInfo {
+    u64 occ, us, them, pawns, K;
+    u64 att;       // squares attacked by the opponent (through our king)
+    u64 pinned;    // our pinned pieces
+    u64 checkers;  // opponent pieces giving check
+    u64 block;     // squares between king and a single sliding checker
+    int ksq;
+};
+
+PF_HD u64 between_on(u64 line, int a, int b) {
+    int lo = a < b ? a : b, hi = a < b ? b : a;
+    return line & ((bit(hi) - 1) ^ ((bit(lo) << 1) - 1));
+}
+
+template <class SL>
+PF_HD void analyse(const Board& b, const SL& sl, PosInfo& pi) {
+    const u64 pawns = b.pawns & kRanks2to7;
+    const u64 occ = pawns | b.knights | b.bishops | b.rooks | b.kings;
+    const u64 us = b.us, them = occ & ~us;
+    const u64 K = b.kings & us;
+    const int ksq = ctz(K);
+    const u64 eB = b.bishops & them, eR = b.rooks & them;
+    const u64 eP = pawns & them, eN = b.knights & them;
+    const u64 occX = occ ^ K;
+    u64 att = ((eP >> 7) & kNotA) | ((eP >> 9) & kNotH);
+    att |= knight_fill(eN) | king_fill(b.kings & them);
+    att |= sl.bishop_multi(eB, occX) | sl.rook_multi(eR, occX);
+
+    u64 pinned = 0, checkers = 0, block = 0;
+    const u64 kf = file_of(ksq), kr = rank_of(ksq), kd = diag_of(ksq), ka = anti_of(ksq);
+    u64 sn = ((kf | kr) & eR) | ((kd | ka) & eB);
+    while (sn) {
+        const int s = ctz(sn);
+        sn &= sn - 1;
+        const u64 sb = bit(s);
+        const u64 line = (kf & sb) ? kf : (kr & sb) ? kr : (kd & sb) ? kd : ka;
+        const u64 btw = between_on(line, ksq, s);
+        const u64 in = btw & occ;
+        if (!in) { checkers |= sb; block |= btw; }
+        else if (!(in & (in - 1))) pinned |= in & us;
+    }
+    checkers |= (knight_fill(K) & eN) | ((((K << 7) & kNotH) | ((K << 9) & kNotA)) & eP);
+    pi.occ = occ; pi.us = us; pi.them = them; pi.pawns = pawns; pi.K = K;
+    pi.att = att; pi.pinned = pinned; pi.checkers = checkers; pi.block = block; pi.ksq = ksq;
+}
+
+// Knight moves of every knight in `n` onto `tgt`: the eight jump directions
+// are translations, so two knights never share a target within a direction.
+PF_HD int knight_count(u64 n, u64 tgt) {
+    const u64 l1 = (n >> 1) & 0x7f7f7f7f7f7f7f7fULL, l2 = (n >> 2) & 0x3f3f3f3f3f3f3f3fULL;
+    const u64 r1 = (n << 1) & 0xfefefefefefefefeULL, r2 = (n << 2) & 0xfcfcfcfcfcfcfcfcULL;
+    return popc((l1 << 16) & tgt) + popc((r1 << 16) & tgt) + popc((l1 >> 16) & tgt) + popc((r1 >> 16) & tgt) +
+           popc((l2 << 8) & tgt) + popc((r2 << 8) & tgt) + popc((l2 >> 8) & tgt) + popc((r2 >> 8) & tgt);
+}
+
+// Slider moves onto `tgt` for bishops-like set B and rook-like set R, with
+// pinned pieces kept on their axis.  Kogge-Stone: one fill per direction.
+template <class SL>
+PF_HD int slider_count(const SL& sl, u64 B, u64 R, u64 occ, u64 tgt, u64 pinned, int ksq) {
+    if constexpr (SL::kSetwise) {
+        const u64 empty = ~occ, free = ~pinned;
+        const u64 rv = R & (free | file_of(ksq)), rh = R & (free | rank_of(ksq));
+        const u64 bd = B & (free | diag_of(ksq)), ba = B & (free | anti_of(ksq));
+        return popc(att_n(rv, empty) & tgt) + popc(att_s(rv, empty) & tgt) +
+               popc(att_e(rh, empty) & tgt) + popc(att_w(rh, empty) & tgt) +
+               popc(att_ne(bd, empty) & tgt) + popc(att_sw(bd, empty) & tgt) +
+               popc(att_nw(ba, empty) & tgt) + popc(att_se(ba, empty) & tgt);
+    } else {
+        int n = 0;
+        const u64 kl[4] = {file_of(ksq), rank_of(ksq), diag_of(ksq), anti_of(ksq)};
+        while (B) {
+            const int s = ctz(B);
+            B &= B - 1;
+            u64 a = sl.bishop(s, occ) & tgt;
+            if (pinned & bit(s)) a &= (kl[2] & bit(s)) ? kl[2] : (kl[3] & bit(s)) ? kl[3] : 0;
+            n += popc(a);
+        }
+        while (R) {
+            const int s = ctz(R);
+            R &= R - 1;
+            u64 a = sl.rook(s, occ) & tgt;
+            if (pinned & bit(s)) a &= (kl[0] & bit(s)) ? kl[0] : (kl[1] & bit(s)) ? kl[1] : 0;
+            n += popc(a);
+        }
+        return n;
+    }
+}
+
+// Number of legal moves (countMoves, MoveGeneratorBitboard.h:2828-3295).
+template <class SL>
+PF_HD u32 count_moves(const Board& b, const SL& sl) {
+    PosInfo pi;
+    analyse(b, sl, pi);
+    const u64 occ = pi.occ, us = pi.us, them = pi.them, empty = ~occ;
+    const u64 P = pi.pawns & us;
+    const u64 Pf = P & ~pi.pinned;
+    const int ep = st_ep(b);
+    int n = popc(king_fill(pi.K) & ~us & ~pi.att);
+
+    if (pi.checkers) {
+        // countMovesOutOfCheck (MoveGeneratorBitboard.h:2828-3002)
+        if (pi.checkers & (pi.checkers - 1)) return (u32)n;
+        const u64 safe = pi.checkers | pi.block;
+        const u64 p1 = (Pf << 8) & empty;
+        n += popc(p1 & safe) + 3 * popc(p1 & safe & kRank8);
+        n += popc((((p1 & ~safe) & kRank3) << 8) & empty & safe);
+        const u64 cl = ((Pf << 7) & kNotH) & them & safe, cr = ((Pf << 9) & kNotA) & them & safe;
+        n += popc(cl) + popc(cr) + 3 * (popc(cl & kRank8) + popc(cr & kRank8));
+        if (ep) {
+            const u64 C = bit(32 + ep - 1);
+            if (C == pi.checkers) n += popc((((C << 1) & kNotA) | ((C >> 1) & kNotH)) & Pf);
+        }
+        n += knight_count(b.knights & us & ~pi.pinned, safe);
+        n += slider_count(sl, b.bishops & us & ~pi.pinned, b.rooks & us & ~pi.pinned, occ, safe, 0, pi.ksq);
+        return (u32)n;
+    }
+
+    const int ksq = pi.ksq;
+    const u64 pinF = pi.pinned & file_of(ksq), pinD = pi.pinned & diag_of(ksq), pinA = pi.pinned & anti_of(ksq);
+    // en passant (MoveGeneratorBitboard.h:3058-3107)
+    if (ep) {
+        const u64 T = bit(40 + ep - 1), C = T >> 8;
+        u64 src = (((C << 1) & kNotA) | ((C >> 1) & kNotH)) & P;
+        const u64 eR = b.rooks & them;
+        while (src) {
+            const u64 s = src & (0 - src);
+            src ^= s;
+            if (s & pi.pinned) {
+                const u64 line = (pinD & s) ? diag_of(ksq) : (pinA & s) ? anti_of(ksq) : (pinF & s) ? file_of(ksq) : rank_of(ksq);
+                n += (T & line) ? 1 : 0;
+            } else {
+                const u64 pro = empty | C | s;
+                n += ((att_e(eR, pro) | att_w(eR, pro)) & pi.K) ? 0 : 1;
+            }
+        }
+    }
+    // pawns (MoveGeneratorBitboard.h:3109-3178)
+    {
+        const u64 p1 = (Pf << 8) & empty;
+        n += popc(p1) + 3 * popc(p1 & kRank8);
+        n += popc(((p1 & kRank3) << 8) & empty);
+        const u64 cl = ((Pf << 7) & kNotH) & them, cr = ((Pf << 9) & kNotA) & them;
+        n += popc(cl) + popc(cr) + 3 * (popc(cl & kRank8) + popc(cr & kRank8));
+        const u64 q1 = ((P & pinF) << 8) & empty;  // file-pinned pawns push along the pin
+        n += popc(q1) + popc(((q1 & kRank3) << 8) & empty);
+        const u64 qd = (((P & pinD) << 9) & kNotA) & them;  // stays on the a1-h8 pin line
+        const u64 qa = (((P & pinA) << 7) & kNotH) & them;  // stays on the a8-h1 pin line
+        n += popc(qd) + popc(qa) + 3 * (popc(qd & kRank8) + popc(qa & kRank8));
+    }
+    // castling (MoveGeneratorBitboard.h:3180-3198)
+    const int r = st_rights(b);
+    n += ((r & 1) && !(occ & kF1G1) && !(pi.att & kF1G1)) ? 1 : 0;
+    n += ((r & 2) && !(occ & kB1D1) && !(pi.att & kC1D1)) ? 1 : 0;
+    n += knight_count(b.knights & us & ~pi.pinned, ~us);
+    n += slider_count(sl, b.bishops & us, b.rooks & us, occ, ~us, pi.pinned, ksq);
+    return (u32)n;
+}
+
+// Legal move generation.  `emit(move)` is called once per legal move; the
+// order is: king, castles, knights, bishops, rooks, pawn pushes, double
+// pushes, captures, en passant.  Perft counts do not depend on it.
+template <class SL, class Emit>
+PF_HD void generate(const Board& b, const SL& sl, const PosInfo& pi, Emit& emit) {
+    const u64 occ = pi.occ, us = pi.us, them = pi.them, empty = ~occ;
+    const int ksq = pi.ksq;
+    const u64 P = pi.pawns & us, Pf = P & ~pi.pinned;
+    {
+        u64 t = king_fill(pi.K) & ~us & ~pi.att;
+        while (t) { emit(mk_move(ksq, ctz(t), kFlQuiet)); t &= t - 1; }
+    }
+    const bool check = pi.checkers != 0;
+    if (check && (pi.checkers & (pi.checkers - 1))) return;
+    const u64 tgt = check ? (pi.checkers | pi.block) : ~us;
+    const u64 free = ~pi.pinned;
+    if (!check) {
+        const int r = st_rights(b);
+        if ((r & 1) && !(occ & kF1G1) && !(pi.att & kF1G1)) emit(mk_move(4, 6, kFlOO));
+        if ((r & 2) && !(occ & kB1D1) && !(pi.att & kC1D1)) emit(mk_move(4, 2, kFlOOO));
+    }
+    {
+        u64 N = b.knights & us & free;
+        while (N) {
+            const int s = ctz(N);
+            N &= N - 1;
+            u64 t = knight_fill(bit(s)) & tgt;
+            while (t) { emit(mk_move(s, ctz(t), kFlQuiet)); t &= t - 1; }
+        }
+    }
+    {
+        const u64 kl[4] = {file_of(ksq), rank_of(ksq), diag_of(ksq), anti_of(ksq)};
+        u64 S = (b.bishops | b.rooks) & us;
+        if (check) S &= free;
+        while (S) {
+            const int s = ctz(S);
+            const u64 sb = bit(s);
+            S &= S - 1;
+            u64 a = 0;
+            if (b.bishops & sb) a |= sl.bishop(s, occ);
+            if (b.rooks & sb) a |= sl.rook(s, occ);
+            a &= tgt;
+            if (pi.pinned & sb) a &= (kl[0] & sb) ? kl[0] : (kl[1] & sb) ? kl[1] : (kl[2] & sb) ? kl[2] : kl[3];
+            while (a) { emit(mk_move(s, ctz(a), kFlQuiet)); a &= a - 1; }
+        }
+    }
+    {
+        const u64 pinF = pi.pinned & file_of(ksq), pinD = pi.pinned & diag_of(ksq), pinA = pi.pinned & anti_of(ksq);
+        const u64 Ppush = check ? Pf : (Pf | (P & pinF));
+        const u64 Pne = check ? Pf : (Pf | (P & pinD));
+        const u64 Pnw = check ? Pf : (Pf | (P & pinA));
+        const u64 p1 = (Ppush << 8) & empty;
+        u64 t = p1 & tgt;
+        while (t) {
+            const int d = ctz(t);
+            t &= t - 1;
+            if (d >= 56) {
+                emit(mk_move(d - 8, d, kFlPromoN)); emit(mk_move(d - 8, d, kFlPromoB));
+                emit(mk_move(d - 8, d, kFlPromoR)); emit(mk_move(d - 8, d, kFlPromoQ));
+            } else {
+                emit(mk_move(d - 8, d, kFlQuiet));
+            }
+        }
+        t = (((check ? (p1 & ~tgt) : p1) & kRank3) << 8) & empty & tgt;
+        while (t) { const int d = ctz(t); t &= t - 1; emit(mk_move(d - 16, d, kFlDouble)); }
+        for (int side = 0; side < 2; side++) {
+            // a pawn pinned on a diagonal captures along it: its NE (a1-h8 pin) or
+            // NW (a8-h1 pin) neighbour lies on the pin line by construction
+            u64 c = side ? (((Pnw << 7) & kNotH) & them & tgt) : (((Pne << 9) & kNotA) & them & tgt);
+            const int delta = side ? 7 : 9;
+            while (c) {
+                const int d = ctz(c);
+                c &= c - 1;
+                if (d >= 56) {
+                    emit(mk_move(d - delta, d, kFlPromoN)); emit(mk_move(d - delta, d, kFlPromoB));
+                    emit(mk_move(d - delta, d, kFlPromoR)); emit(mk_move(d - delta, d, kFlPromoQ));
+                } else {
+                    emit(mk_move(d - delta, d, kFlQuiet));
+                }
+            }
+        }
+        const int ep = st_ep(b);
+        if (ep) {
+            const u64 T = bit(40 + ep - 1), C = T >> 8;
+            u64 src = (((C << 1) & kNotA) | ((C >> 1) & kNotH)) & P;
+            const u64 eR = b.rooks & them;
+            while (src) {
+                const u64 s = src & (0 - src);
+                src ^= s;
+                bool ok;
+                if (check) {
+                    ok = (C == pi.checkers) && !(s & pi.pinned);
+                } else if (s & pi.pinned) {
+                    const u64 line = (pinD & s) ? diag_of(ksq) : (pinA & s) ? anti_of(ksq) : (pinF & s) ? file_of(ksq) : rank_of(ksq);
+                    ok = (T & line) != 0;
+                } else {
+                    const u64 pro = empty | C | s;
+                    ok = !((att_e(eR, pro) | att_w(eR, pro)) & pi.K);
+                }
+                if (ok) emit(mk_move(ctz(s), 40 + ep - 1, kFlEP));
+            }
+        }
+    }
+}
+
+// Apply a legal move and return the child in ITS side-to-move frame
+// (makeMove, MoveGeneratorBitboard.h:3310-3560, followed by the colour flip).
+// Castling rights: any move from or to e1/h1/a1 (ours) or e8/h8/a8 (theirs)
+// clears the matching rights -- the same rule as the reference's king-move,
+// rook-move and rook-capture updates on every reachable position.  The child
+// gets an en-passant file only if an enemy pawn stands next to the
+// double-pushed pawn (EXACT_EN_PASSENT_FLAGGING, :3528-3547).
+PF_HD Board make_move(const Board& b, u32 m) {
+    const int from = m & 63, to = (m >> 6) & 63, fl = (m >> 12) & 15;
+    const u64 src = bit(from), dst = bit(to), mv = src | dst;
+    u64 P = b.pawns & kRanks2to7 & ~dst;
+    u64 N = b.knights & ~dst, B = b.bishops & ~dst, R = b.rooks & ~dst, Kg = b.kings;
+    u64 us = (b.us ^ src) | dst;
+    P ^= mv & (0 - ((P >> from) & 1));
+    N ^= mv & (0 - ((N >> from) & 1));
+    B ^= mv & (0 - ((B >> from) & 1));
+    R ^= mv & (0 - ((R >> from) & 1));
+    Kg ^= mv & (0 - ((Kg >> from) & 1));
+    if (fl & 8) {
+        P &= ~dst;
+        N |= (fl == kFlPromoN) ? dst : 0;
+        B |= (fl == kFlPromoB || fl == kFlPromoQ) ? dst : 0;
+        R |= (fl == kFlPromoR || fl == kFlPromoQ) ? dst : 0;
+    }
+    if (fl == kFlEP) P &= ~(dst >> 8);
+    if (fl == kFlOO) { R ^= 0xA0ULL; us ^= 0xA0ULL; }
+    if (fl == kFlOOO) { R ^= 0x09ULL; us ^= 0x09ULL; }
+    int rights = (int)(b.pawns & 15);
+    const int clr = (int)(((mv >> 4) & 1) * 3) | (int)((mv >> 7) & 1) | (int)((mv & 1) << 1) |
+                    (int)(((mv >> 60) & 1) * 12) | (int)(((mv >> 63) & 1) << 2) | (int)(((mv >> 56) & 1) << 3);
+    rights &= ~clr;
+    int ep = 0;
+    if (fl == kFlDouble) {
+        const u64 adj = ((dst << 1) & kNotA) | ((dst >> 1) & kNotH);
+        if (adj & P & ~us) ep = (from & 7) + 1;
+    }
+    const u64 occ = P | N | B | R | Kg;
+    Board c;
+    c.us = flip(occ & ~us);
+    c.pawns = flip(P) | (u64)(((rights >> 2) & 3) | ((rights & 3) << 2)) | ((u64)ep << 4);
+    c.knights = flip(N);
+    c.bishops = flip(B);
+    c.rooks = flip(R);
+    c.kings = flip(Kg);
+    return c;
+}
+
+PF_HD Board with_count(Board b, u32 n) {
+    b.pawns = (b.pawns & ~(0xFFULL << 56)) | ((u64)n << 56);
+    return b;
+}
+
+}  // namespace pf
