@@ -125,6 +125,9 @@ pf_status pf_apply_uci(pf_pos48 *pos, const char *uci);
  * are compile-time). */
 pf_status pf_set_backend(pf_ctx *ctx, int backend);
 pf_status pf_get_stats(pf_ctx *ctx, pf_stats *out);
+/* Wait for all device work of this context (the reference's cudaDeviceSynchronize
+ * calls around its launches, launcher.h:1490-1492). */
+pf_status pf_sync(pf_ctx *ctx);
 pf_status pf_reset_stats(pf_ctx *ctx);
 
 /* Seeded random-playout positions (workload generator, no reference

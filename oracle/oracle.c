@@ -24,7 +24,7 @@ typedef uint16_t u16;
 typedef uint8_t u8;
 
 #ifdef ORC_OPCOUNT
-static __thread u64 g_ops;
+static __thread u64 g_ops __attribute__((tls_model("initial-exec")));
 #define OPS(n) (g_ops += (u64)(n))
 #else
 #define OPS(n) ((void)0)
@@ -239,7 +239,7 @@ static u64 find_plain_magic(u64 mask, int bits, u64 *table, int is_rook, int sq)
         u64 pro = ~occ[i];
         att[i] = is_rook ? rook_ks(BIT(sq), pro) : bishop_ks(BIT(sq), pro);
     }
-    for (;;) {
+    for (long tries = 0; tries < 100000000L; tries++) {
         u64 m = xs64(&rng_state_magic) & xs64(&rng_state_magic) & xs64(&rng_state_magic);
         memset(table, 0, sizeof(u64) << bits);
         int i;
@@ -250,6 +250,7 @@ static u64 find_plain_magic(u64 mask, int bits, u64 *table, int is_rook, int sq)
         }
         if (i == nc) return m;
     }
+    return 0; /* not found: orc_init reports failure */
 }
 
 /* MoveGeneratorBitboard.h:1055-1076 + Magics.cpp:42-103 with a precalculated
@@ -318,6 +319,7 @@ int orc_init(void) {
     for (int sq = 0; sq < 64; sq++) {
         T_plain_rmagic[sq] = find_plain_magic(T_rook_mask[sq], 12, T_plain_rtab[sq], 1, sq);
         T_plain_bmagic[sq] = find_plain_magic(T_bishop_mask[sq], 9, T_plain_btab[sq], 0, sq);
+        if (!T_plain_rmagic[sq] || !T_plain_bmagic[sq]) return -4;
     }
     {
         int k = 0;
@@ -1350,7 +1352,7 @@ int orc_playouts(uint64_t seed, int n, int min_plies, int max_plies, int min_mov
 
 /* ---- op model -------------------------------------------------------------- */
 #ifdef ORC_OPCOUNT
-static u64 g_leaves;
+static __thread u64 g_leaves __attribute__((tls_model("initial-exec")));
 /* breadth-first path: make+count on every non-root node; generate on every
  * node with remaining depth >= 2 (the last interior ply). */
 static void opcount_rec(const orc_pos *pos, int depth) {

@@ -68,7 +68,7 @@ _lib = None
 # every symbol the C ABI declares (checked by the CPU test suite)
 EXPORTS = ["pf_init", "pf_destroy", "pf_strerror", "pf_fen_to_pos", "pf_pos_to_fen", "pf_perft",
            "pf_perft_batch", "pf_perft_batch_mode", "pf_divide", "pf_move_to_uci", "pf_apply_uci",
-           "pf_set_backend", "pf_get_stats", "pf_reset_stats", "pf_playouts", "pf_count_moves_host"]
+           "pf_set_backend", "pf_get_stats", "pf_reset_stats", "pf_sync", "pf_playouts", "pf_count_moves_host"]
 
 
 def load_library(path=LIB_PATH):
@@ -98,6 +98,7 @@ def load_library(path=LIB_PATH):
         "pf_set_backend": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
         "pf_get_stats": (ctypes.c_int, [ctypes.c_void_p, P(Stats)]),
         "pf_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
+        "pf_sync": (ctypes.c_int, [ctypes.c_void_p]),
         "pf_playouts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_void_p]),
         "pf_count_moves_host": (ctypes.c_int, [P(Pos48)]),
@@ -237,6 +238,9 @@ class Engine:
         s = Stats()
         _check(self._lib.pf_get_stats(self._ctx, ctypes.byref(s)), "stats")
         return s.as_dict()
+
+    def sync(self):
+        _check(self._lib.pf_sync(self._ctx), "sync")
 
     def reset_stats(self):
         _check(self._lib.pf_reset_stats(self._ctx), "reset_stats")

@@ -257,6 +257,15 @@ pf_status pf_get_stats(pf_ctx* ctx, pf_stats* out) {
     return PF_OK;
 }
 
+pf_status pf_sync(pf_ctx* ctx) {
+    if (!ctx) return PF_EINVAL;
+    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    if (hipSetDevice(ctx->eng.device) != hipSuccess || hipStreamSynchronize(ctx->eng.stream) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess)
+        return PF_EDEVICE;
+    return PF_OK;
+}
+
 pf_status pf_reset_stats(pf_ctx* ctx) {
     if (!ctx) return PF_EINVAL;
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);

@@ -3,6 +3,7 @@ infrastructure only; the product never imports this module."""
 import ctypes
 import os
 import subprocess
+import threading
 
 import numpy as np
 
@@ -75,16 +76,19 @@ def _load(path):
 class Oracle:
     _lib = None
     _lib_ops = None
+    _lock = threading.Lock()  # orc_init is not re-entrant
 
     def __init__(self):
-        if Oracle._lib is None:
-            Oracle._lib = _load(LIB)
+        with Oracle._lock:
+            if Oracle._lib is None:
+                Oracle._lib = _load(LIB)
         self.lib = Oracle._lib
 
     @property
     def ops_lib(self):
-        if Oracle._lib_ops is None:
-            Oracle._lib_ops = _load(LIB_OPS)
+        with Oracle._lock:
+            if Oracle._lib_ops is None:
+                Oracle._lib_ops = _load(LIB_OPS)
         return Oracle._lib_ops
 
     def set_backend(self, b):

@@ -142,3 +142,22 @@ def test_movegen_slider_tables_vs_oracle(oracle, mg):
             for b in (0, 1, 2, 3):
                 assert mg.mg_attacks(b, 1, sq, occ) == oracle.rook_attacks(0, sq, occ)
                 assert mg.mg_attacks(b, 0, sq, occ) == oracle.bishop_attacks(0, sq, occ)
+
+
+def test_workload_positions_match_golden(pg):
+    """the product's seeded workloads reproduce the oracle-generated fixtures"""
+    import json
+    from perft_gpu_amd import workloads as W
+    with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
+        g = json.load(f)
+    c2 = W.positions(W.C2)
+    assert [["%016x" % int(x) for x in r] for r in c2] == g["c2"]["positions"]
+    c3 = W.positions(W.C3)
+    assert [["%016x" % int(x) for x in r] for r in c3] == g["c3"]["positions"]
+    c5 = W.positions(W.C5)
+    assert len(c5) == 4096 and [["%016x" % int(x) for x in r] for r in c5[:64]] == g["c5"]["positions_first64"]
+    reqs = W.serving_requests(W.C4, limit=300)
+    for (t, d, s), want in zip(reqs, g["c4"]["first300"]):
+        assert (t, d, s) == (want["t"], want["depth"], want["seed"])
+        assert ["%016x" % int(x) for x in W.request_position(s)] == want["pos"]
+    assert sum(g["c2"]["perft"]["5"]) == W.C2_LEAVES

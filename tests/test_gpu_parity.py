@@ -42,3 +42,92 @@ def test_batch_vs_oracle(pg, oracle, engine, mode):
         got = engine.perft_batch(arr, depth, mode=mode)
         want = oracle.perft_batch(arr, depth, nthreads=8)
         assert np.array_equal(got, want), (depth, mode)
+
+
+def _golden():
+    with open(os.path.join(GOLDEN, "workloads.json")) as f:
+        return json.load(f)
+
+
+def _rows(pg, hexrows):
+    arr = np.zeros(len(hexrows), dtype=pg.POS_DTYPE)
+    for i, r in enumerate(hexrows):
+        arr[i] = tuple(int(x, 16) for x in r)
+    return arr
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_c2_full_size_vs_golden(pg, engine, mode):
+    """the full C2 workload (256 positions, perft 5, 9.0e9 leaves)"""
+    g = _golden()["c2"]
+    pos = _rows(pg, g["positions"])
+    for d in (3, 5):
+        got = engine.perft_batch(pos, d, mode=mode)
+        assert [int(x) for x in got] == g["perft"][str(d)], (d, mode)
+
+
+@pytest.mark.parametrize("backend", ["plain", "fancy", "byte"])
+def test_c2_backends_vs_golden(pg, engine, backend):
+    g = _golden()["c2"]
+    pos = _rows(pg, g["positions"])
+    engine.set_backend(backend)
+    try:
+        got = engine.perft_batch(pos, 5, mode=2)
+    finally:
+        engine.set_backend("ks")
+    assert [int(x) for x in got] == g["perft"]["5"]
+
+
+def test_c3_c5_positions_vs_golden(pg, engine):
+    g = _golden()
+    c3 = _rows(pg, g["c3"]["positions"])
+    assert [int(engine.perft_batch(c3, d)[0]) for d in range(1, 7)] == [g["c3"]["perft"][str(d)][0] for d in range(1, 7)]
+    c5 = _rows(pg, g["c5"]["positions_first64"])
+    assert [int(x) for x in engine.perft_batch(c5, 5)] == g["c5"]["perft_first64"]["5"]
+    assert [int(x) for x in engine.perft_batch(c5[:8], 6)] == g["c5"]["perft_first8_d6"]
+
+
+def test_edge_cases(pg, oracle, engine):
+    # empty batch, depth 0, mate and stalemate roots, the 218-move position
+    assert len(engine.perft_batch(np.zeros(0, dtype=pg.POS_DTYPE), 3)) == 0
+    start = pg.fen_to_pos(pg.START_FEN)
+    assert engine.perft(start, 0) == 1
+    mate = pg.fen_to_pos("rnb1kbnr/pppp1ppp/8/4p3/6Pq/5P2/PPPPP2P/RNBQKBNR w KQkq - 1 3")
+    stale = pg.fen_to_pos("7k/5Q2/6K1/8/8/8/8/8 b - - 0 1")
+    m218 = pg.fen_to_pos("3Q4/1Q4Q1/4Q3/2Q4R/Q4Q2/3Q4/1Q4Rp/1K1BBNNk w - - 0 1")
+    for d in (1, 2, 3, 4):
+        got = engine.perft_batch([mate, stale, start], d, mode=2)
+        assert list(map(int, got[:2])) == [0, 0]
+    assert engine.perft(m218, 1) == 218
+    assert engine.perft(m218, 3) == oracle.perft_fen("3Q4/1Q4Q1/4Q3/2Q4R/Q4Q2/3Q4/1Q4Rp/1K1BBNNk w - - 0 1", 3)
+    with pytest.raises(pg.PerftError):
+        engine.perft(start, 17)
+
+
+def test_chunking_small_arena(pg, oracle):
+    """a 4 MiB arena forces depth-first chunking; counts must not change"""
+    eng = pg.Engine(backend="ks", arena_bytes=4 << 20)
+    try:
+        pos = pg.playouts(777, 16, 10, 40)
+        want = oracle.perft_batch(pos, 5, nthreads=8)
+        got = eng.perft_batch(pos, 5, mode=pg.MODE_LEVELS)
+        assert np.array_equal(got, want)
+        assert eng.stats()["chunks"] > 0
+        # persistent mode cannot chunk: it reports ENOMEM, AUTO falls back to levels
+        with pytest.raises(pg.PerftError) as ei:
+            eng.perft_batch(pos, 5, mode=pg.MODE_PERSISTENT)
+        assert ei.value.status == pg.PF_ENOMEM
+        assert np.array_equal(eng.perft_batch(pos, 5, mode=pg.MODE_AUTO), want)
+    finally:
+        eng.close()
+
+
+def test_divide(pg, oracle, engine):
+    pos = pg.fen_to_pos("r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq - 0 1")
+    div = engine.divide(pos, 4)
+    assert len(div) == 48 and sum(c for _, c in div) == 4085603
+    o = oracle.pos("r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq - 0 1")
+    want = {}
+    for m, c in oracle.divide(o, 4):
+        want[pg.move_to_uci(m & 0x0FFF | ((m >> 12) & 0xB) << 12 if (m >> 12) & 8 else m & 0x0FFF)] = c
+    assert dict(div) == want
