@@ -130,6 +130,11 @@ pf_status pf_get_stats(pf_ctx *ctx, pf_stats *out);
 pf_status pf_sync(pf_ctx *ctx);
 pf_status pf_reset_stats(pf_ctx *ctx);
 
+/* Diagnostics (no reference counterpart): measure a ceiling on this device.
+ * kind 0: int32 VALU lane-ops/s; 1: streaming copy, read+write bytes/s over a
+ * `bytes` buffer; 2: random 16-byte read-modify-writes/s over a `bytes` table. */
+pf_status pf_probe(pf_ctx *ctx, int kind, uint64_t bytes, uint64_t iters, double *out);
+
 /* Seeded random-playout positions (workload generator, no reference
  * counterpart): xorshift64(13,7,17) stream; per candidate, a length uniform in
  * [min_plies, max_plies], each ply a uniform pick from the legal moves in

@@ -68,7 +68,8 @@ _lib = None
 # every symbol the C ABI declares (checked by the CPU test suite)
 EXPORTS = ["pf_init", "pf_destroy", "pf_strerror", "pf_fen_to_pos", "pf_pos_to_fen", "pf_perft",
            "pf_perft_batch", "pf_perft_batch_mode", "pf_divide", "pf_move_to_uci", "pf_apply_uci",
-           "pf_set_backend", "pf_get_stats", "pf_reset_stats", "pf_sync", "pf_playouts", "pf_count_moves_host"]
+           "pf_set_backend", "pf_get_stats", "pf_reset_stats", "pf_sync", "pf_probe", "pf_playouts",
+           "pf_count_moves_host"]
 
 
 def load_library(path=LIB_PATH):
@@ -99,6 +100,8 @@ def load_library(path=LIB_PATH):
         "pf_get_stats": (ctypes.c_int, [ctypes.c_void_p, P(Stats)]),
         "pf_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
         "pf_sync": (ctypes.c_int, [ctypes.c_void_p]),
+        "pf_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
+                                    ctypes.POINTER(ctypes.c_double)]),
         "pf_playouts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_void_p]),
         "pf_count_moves_host": (ctypes.c_int, [P(Pos48)]),
@@ -241,6 +244,12 @@ class Engine:
 
     def sync(self):
         _check(self._lib.pf_sync(self._ctx), "sync")
+
+    def probe(self, kind, nbytes=0, iters=0):
+        """measured ceiling: 0 int32 lane-ops/s, 1 copy bytes/s, 2 random 16-B RMW/s"""
+        out = ctypes.c_double()
+        _check(self._lib.pf_probe(self._ctx, kind, nbytes, iters, ctypes.byref(out)), "probe")
+        return out.value
 
     def reset_stats(self):
         _check(self._lib.pf_reset_stats(self._ctx), "reset_stats")

@@ -266,6 +266,12 @@ pf_status pf_sync(pf_ctx* ctx) {
     return PF_OK;
 }
 
+pf_status pf_probe(pf_ctx* ctx, int kind, uint64_t bytes, uint64_t iters, double* out) {
+    if (!ctx || !out) return PF_EINVAL;
+    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    return ctx->eng.probe(kind, bytes, iters, out);
+}
+
 pf_status pf_reset_stats(pf_ctx* ctx) {
     if (!ctx) return PF_EINVAL;
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);

@@ -48,6 +48,7 @@ struct Engine {
     pf_status perft_persistent(const pf_pos48* pos, int n, int depth, u64* out);
     pf_status init_tt(u64 bytes, u64 seed);
     void load_tuning(const char* path);
+    pf_status probe(int kind, u64 bytes, u64 iters, double* out);
 
   private:
     char* alloc(u64 bytes);
