@@ -83,6 +83,10 @@ void Engine::destroy() {
     hipFree(d_sums);
     hipFree(d_wsums);
     hipFree(d_pctl);
+    hipFree(d_tt);
+    hipFree(d_z);
+    hipFree(d_ttstats);
+    d_tt = nullptr; d_z = nullptr; d_ttstats = nullptr; tt_buckets = 0;
     hipHostFree(h_pinned);
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
@@ -222,6 +226,7 @@ pf_status Engine::perft_levels(const pf_pos48* pos, int n, int d, u64* out) {
         for (int i = 0; i < n; i++) out[i] = 1;
         return PF_OK;
     }
+    if (d_tt && d >= 3) return perft_levels_tt(pos, n, d, out);
     PF_HIP(hipSetDevice(device));
     pf_status s = ensure_sums((u64)n);
     if (s != PF_OK) return s;
@@ -301,10 +306,6 @@ pf_status Engine::perft_persistent(const pf_pos48* pos, int n, int d, u64* out) 
     return PF_OK;
 }
 
-pf_status Engine::init_tt(u64 bytes, u64 seed) {
-    (void)seed;
-    return bytes ? PF_EINVAL : PF_OK;
-}
 
 void Engine::load_tuning(const char* path) { (void)path; }
 

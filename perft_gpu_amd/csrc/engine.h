@@ -7,8 +7,11 @@
 #include "../../include/perft_gpu_amd.h"
 #include "convert.h"
 #include "kernels.h"
+#include "tt_kernels.h"
 
 namespace pf {
+
+constexpr u64 kAlignTT = 256;
 
 struct Engine {
     std::recursive_mutex mu;
@@ -41,6 +44,23 @@ struct Engine {
 
     int depth = 0;  // depth of the current run
 
+    // transposition table (tt.h): 2^k buckets of two 16-byte entries
+    uint4* d_tt = nullptr;
+    u64 tt_buckets = 0;
+    ZTables h_z;
+    ZTables* d_z = nullptr;
+    unsigned long long* d_ttstats = nullptr;  // probes, hits, stores
+    int tt_grid[4] = {};
+
+    struct TLevel {  // one breadth-first level with TT bookkeeping (80 B/node)
+        Board* b;
+        u32* root;
+        u64* k1;
+        u64* k2;
+        u32* parent;
+        unsigned long long* acc;
+    };
+
     pf_status init(const pf_config& cfg);
     void destroy();
     pf_status ensure_sums(u64 n);
@@ -58,6 +78,13 @@ struct Engine {
     pf_status range_count(const Board* b, u64 n, u64* total);
     pf_status upload_roots(const pf_pos48* pos, int n, Board** roots, u32** rid);
     pf_status descend(const Board* lvl, const u32* rts, u64 n, u64 n_next, int L);
+    bool alloc_tlevel(u64 n, TLevel* lv);
+    pf_status expand_tt(int mode, const TLevel& lv, u64 off, u64 m, const TLevel* nx, u64 cap, int L,
+                        u64* written, u64* next_total);
+    pf_status descend_tt(const TLevel& lv, u64 off, u64 m, u64 n_next, int L);
+  public:
+    pf_status perft_levels_tt(const pf_pos48* pos, int n, int depth, u64* out);
+    pf_status tt_clear();
 };
 
 }  // namespace pf

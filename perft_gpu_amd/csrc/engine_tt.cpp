@@ -1,0 +1,212 @@
+// engine_tt.cpp -- level driver with the transposition table.
+//
+// Downsweep: every expand-and-write level probes its children's buckets and
+// stores only the misses, each with both keys and its parent's index; the
+// last interior level is expanded in leaf mode into per-node accumulators.
+// Upsweep (on the way back up, level by level and chunk by chunk): every
+// finished node adds its exact count to its parent and is stored in the
+// table at its remaining depth.  Hits never change a count: the stored value
+// is the node's exact perft, verified by a second 64-bit key.  This replaces
+// the reference's per-depth table set (launcher.h:62-126, 155-320) and the
+// hash kernels of perft_bb.h:664-961.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "engine.h"
+
+namespace pf {
+
+#define PF_HIP(x)                                 \
+    do {                                          \
+        if ((x) != hipSuccess) return PF_EDEVICE; \
+    } while (0)
+
+static constexpr u64 kTNodeBytes = sizeof(Board) + 4 + 8 + 8 + 4 + 8;
+
+pf_status Engine::init_tt(u64 bytes, u64 seed) {
+    zobrist_tables(h_z, seed);
+    if (!bytes) return PF_OK;
+    u64 buckets = 1;
+    while (buckets * 2 * 32 <= bytes) buckets *= 2;
+    if (buckets < 1024) return PF_EINVAL;
+    PF_HIP(hipSetDevice(device));
+    if (hipMalloc(&d_tt, buckets * 32) != hipSuccess) { d_tt = nullptr; return PF_ENOMEM; }
+    tt_buckets = buckets;
+    PF_HIP(hipMalloc(&d_z, sizeof(ZTables)));
+    PF_HIP(hipMemcpy(d_z, &h_z, sizeof(ZTables), hipMemcpyHostToDevice));
+    PF_HIP(hipMalloc(&d_ttstats, 4 * sizeof(unsigned long long)));
+    PF_HIP(hipMemset(d_ttstats, 0, 4 * sizeof(unsigned long long)));
+    for (int b = 0; b < 4; b++) tt_grid[b] = expand_tt_grid(b, n_cu);
+    return tt_clear();
+}
+
+pf_status Engine::tt_clear() {
+    if (!d_tt) return PF_OK;
+    PF_HIP(hipMemsetAsync(d_tt, 0, tt_buckets * 32, stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    return PF_OK;
+}
+
+bool Engine::alloc_tlevel(u64 n, TLevel* lv) {
+    lv->b = reinterpret_cast<Board*>(alloc(n * sizeof(Board)));
+    lv->root = reinterpret_cast<u32*>(alloc(n * 4));
+    lv->k1 = reinterpret_cast<u64*>(alloc(n * 8));
+    lv->k2 = reinterpret_cast<u64*>(alloc(n * 8));
+    lv->parent = reinterpret_cast<u32*>(alloc(n * 4));
+    lv->acc = reinterpret_cast<unsigned long long*>(alloc(n * 8));
+    return lv->b && lv->root && lv->k1 && lv->k2 && lv->parent && lv->acc;
+}
+
+pf_status Engine::expand_tt(int mode, const TLevel& lv, u64 off, u64 m, const TLevel* nx, u64 cap, int L,
+                            u64* written, u64* next_total) {
+    if (!m) {
+        if (written) *written = 0;
+        if (next_total) *next_total = 0;
+        return PF_OK;
+    }
+    if (m > 0xFFFFFFFFull || off + m > 0xFFFFFFFFull) return PF_ENOMEM;
+    PF_HIP(hipMemsetAsync(d_ctrl, 0, 4 * sizeof(u32), stream));
+    PF_HIP(hipMemsetAsync(d_total, 0, 8, stream));
+    TTArgs a = {};
+    a.in = lv.b + off;
+    a.in_root = lv.root + off;
+    a.in_k1 = lv.k1 + off;
+    a.in_k2 = lv.k2 + off;
+    a.n_in = (u32)m;
+    a.in_base = (u32)off;
+    a.in_acc = lv.acc + off;
+    if (mode == kWrite) {
+        a.out = nx->b;
+        a.out_root = nx->root;
+        a.out_k1 = nx->k1;
+        a.out_k2 = nx->k2;
+        a.out_parent = nx->parent;
+        a.out_acc = nx->acc;
+        a.out_cap = cap;
+        a.total = d_total;
+    }
+    a.out_count = d_ctrl + 2;
+    a.tt = d_tt;
+    a.tt_mask = tt_buckets - 1;
+    a.child_depth = depth - (L + 1);
+    a.mode = mode;
+    a.tile_ctr = d_ctrl + 1;
+    a.err = d_ctrl + 0;
+    a.stats = d_ttstats;
+    if (timing) PF_HIP(hipEventRecord(ev0, stream));
+    PF_HIP(launch_expand_tt(backend, d_tab, d_z, a, tt_grid[backend], stream));
+    if (timing) PF_HIP(hipEventRecord(ev1, stream));
+    st.kernel_launches++;
+    st.levels_run++;
+    PF_HIP(hipMemcpyAsync(h_pinned, d_ctrl, 4 * sizeof(u32), hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipMemcpyAsync(h_pinned + 4, d_total, 8, hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    const u32* c = reinterpret_cast<const u32*>(h_pinned);
+    if (c[0]) return PF_ENOMEM;
+    if (written) *written = c[2];
+    if (next_total) *next_total = h_pinned[4];
+    if (timing) {
+        float ms = 0;
+        PF_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+        st.gpu_ms += ms;
+        run_expand_ms += ms;
+        run_expand_launches++;
+    }
+    return PF_OK;
+}
+
+// Parents [off, off + m) of level `lv` (level L) have n_next children in all.
+pf_status Engine::descend_tt(const TLevel& lv, u64 off, u64 m, u64 n_next, int L) {
+    if (!m) return PF_OK;
+    if (depth - L == 2) return expand_tt(kLeaf, lv, off, m, nullptr, 0, L, nullptr, nullptr);
+    const u64 need = n_next * kTNodeBytes + 8 * kAlignTT;
+    const u64 free_b = arena_bytes - arena_top;
+    if (need <= free_b && n_next <= 0xFFFFFFFFull) {
+        char* mark = arena + arena_top;
+        TLevel nx;
+        if (!alloc_tlevel(n_next, &nx)) { release(mark); return PF_ENOMEM; }
+        u64 written = 0, n_next2 = 0;
+        pf_status s = expand_tt(kWrite, lv, off, m, &nx, n_next, L, &written, &n_next2);
+        if (s == PF_OK) s = descend_tt(nx, 0, written, n_next2, L + 1);
+        if (s == PF_OK && written) {
+            if (launch_upsweep(nx.parent, nx.k1, nx.k2, nx.acc, (u32)written, lv.acc, d_tt, tt_buckets - 1,
+                               depth - (L + 1), d_ttstats + 2, stream) != hipSuccess)
+                s = PF_EDEVICE;
+            st.kernel_launches++;
+        }
+        release(mark);
+        return s;
+    }
+    if (m == 1) return PF_ENOMEM;
+    st.chunks++;
+    const u64 pieces = std::max<u64>(2, (need + free_b - 1) / std::max<u64>(free_b / 2, 1) + 1);
+    const u64 step = (m + pieces - 1) / pieces;
+    for (u64 a = off; a < off + m; a += step) {
+        const u64 len = std::min(step, off + m - a);
+        u64 nn = 0;
+        pf_status s = range_count(lv.b + a, len, &nn);
+        if (s != PF_OK) return s;
+        s = descend_tt(lv, a, len, nn, L);
+        if (s != PF_OK) return s;
+    }
+    return PF_OK;
+}
+
+pf_status Engine::perft_levels_tt(const pf_pos48* pos, int n, int d, u64* out) {
+    PF_HIP(hipSetDevice(device));
+    pf_status s = ensure_sums((u64)n);
+    if (s != PF_OK) return s;
+    depth = d;
+    run_expand_ms = 0;
+    run_expand_launches = 0;
+    char* mark = arena + arena_top;
+    auto fail = [&](pf_status e) { release(mark); return e; };
+    TLevel r;
+    if (!alloc_tlevel((u64)n, &r)) return fail(PF_ENOMEM);
+    std::vector<Board> hb((size_t)n);
+    std::vector<u32> hr((size_t)n);
+    std::vector<u64> h1((size_t)n), h2((size_t)n);
+    for (int i = 0; i < n; i++) {
+        const int black = pos_black_to_move(pos[i]);
+        hb[(size_t)i] = to_board(pos[i]);
+        hr[(size_t)i] = (u32)i | ((u32)black << 31);
+        h1[(size_t)i] = zobrist_full(h_z.k[0], hb[(size_t)i], black);
+        h2[(size_t)i] = zobrist_full(h_z.k[1], hb[(size_t)i], black);
+    }
+    if (hipMemcpyAsync(r.b, hb.data(), (size_t)n * sizeof(Board), hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipMemcpyAsync(r.root, hr.data(), (size_t)n * 4, hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipMemcpyAsync(r.k1, h1.data(), (size_t)n * 8, hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipMemcpyAsync(r.k2, h2.data(), (size_t)n * 8, hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipMemsetAsync(d_total, 0, 8, stream) != hipSuccess)
+        return fail(PF_EDEVICE);
+    if (launch_roots_tt(backend, d_tab, r.b, r.k1, r.k2, (u32)n, d, d_tt, tt_buckets - 1, r.acc, d_total, stream) !=
+        hipSuccess)
+        return fail(PF_EDEVICE);
+    st.kernel_launches++;
+    if (hipMemcpyAsync(h_pinned + 4, d_total, 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return fail(PF_EDEVICE);
+    const u64 n1 = h_pinned[4];
+    s = descend_tt(r, 0, (u64)n, n1, 0);
+    if (s != PF_OK) return fail(s);
+    // the roots themselves, at full depth (answers repeated requests at once)
+    if (launch_upsweep(nullptr, r.k1, r.k2, r.acc, (u32)n, nullptr, d_tt, tt_buckets - 1, d, d_ttstats + 2, stream) !=
+        hipSuccess)
+        return fail(PF_EDEVICE);
+    unsigned long long ts[4];
+    if (hipMemcpyAsync(out, r.acc, (size_t)n * 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipMemcpyAsync(ts, d_ttstats, sizeof(ts), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return fail(PF_EDEVICE);
+    st.tt_probes = ts[0];
+    st.tt_hits = ts[1];
+    st.tt_stores = ts[2];
+    st.last_expand_ms = run_expand_launches ? run_expand_ms / (double)run_expand_launches : 0.0;
+    st.last_expand_launches = run_expand_launches;
+    release(mark);
+    return PF_OK;
+}
+
+}  // namespace pf

@@ -22,6 +22,7 @@
 #include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
+#include "backends_dev.h"
 #include "kernels.h"
 
 namespace pf {
@@ -42,29 +43,6 @@ __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
     for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
 }
-
-template <int B>
-struct SlOf;
-template <>
-struct SlOf<kKS> {
-    using T = KSBackend;
-    __device__ static T make(const SliderTables*) { return T{}; }
-};
-template <>
-struct SlOf<kPlain> {
-    using T = PlainBackend;
-    __device__ static T make(const SliderTables* t) { return T{t->plain_b, t->plain_r, t->plain_btab, t->plain_rtab}; }
-};
-template <>
-struct SlOf<kFancy> {
-    using T = FancyBackend;
-    __device__ static T make(const SliderTables* t) { return T{t->fancy_b, t->fancy_r, t->fancy_tab}; }
-};
-template <>
-struct SlOf<kByte> {
-    using T = ByteBackend;
-    __device__ static T make(const SliderTables* t) { return T{t->fancy_b, t->fancy_r, t->byte_tab, t->byte_r, t->byte_b}; }
-};
 
 // Byte backend: the 148 KB lookup blob and the 4 KB of magic entries are copied
 // into LDS once per workgroup; the workgroup is persistent, so the copy is

@@ -1,0 +1,325 @@
+// tt.hip -- breadth-first perft with the transposition table.
+//
+// Same tile structure as kernels.hip (parents in LDS, their legal moves in an
+// LDS list, one (parent, move) pair per lane), plus:
+//   * every child carries two incremental Zobrist keys (tt.h) and the index
+//     of its parent;
+//   * expand-and-write: each child (remaining depth >= 2) probes its bucket;
+//     a hit adds the stored count to the parent's accumulator and the child is
+//     not materialised (replaces makemove_and_count_moves_single_level_hash128b,
+//     perft_bb.h:664-786); misses are compacted with a wave ballot and written
+//   * leaf: each parent's accumulator receives perft(parent, 2)
+//   * upsweep: a finished level adds its counts into the parents' accumulators
+//     and stores them in the table (calcPerftNFromPerftNminus1_hash128b,
+//     perft_bb.h:854-961)
+#include <hip/hip_runtime.h>
+
+#include "tt_kernels.h"
+
+namespace pf {
+
+constexpr int kTWG = 256;
+constexpr int kTTP = 128;   // parents per tile
+constexpr int kTCAP = 4096; // LDS move-list window
+
+__device__ __forceinline__ u32 t_wave_incl_scan(u32 v, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const u32 o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long t_wave_sum64(unsigned long long v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+__device__ __forceinline__ bool tt_probe(const uint4* tt, u64 mask, u64 k1, u64 k2, int depth, u64* val) {
+    const uint4* bkt = tt + 2 * (k1 & mask);
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const uint4 e = bkt[s];
+        const u64 w0 = (u64)e.x | ((u64)e.y << 32), w1 = (u64)e.z | ((u64)e.w << 32);
+        if ((w0 ^ w1) == k2 && (int)(w1 >> 56) == depth) {
+            *val = w1 & kCountMask;
+            return true;
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ void tt_store(uint4* tt, u64 mask, u64 k1, u64 k2, int depth, u64 val) {
+    if (val > kCountMask) return;
+    uint4* bkt = tt + 2 * (k1 & mask);
+    const uint4 e0 = bkt[0];
+    const u64 o1 = (u64)e0.z | ((u64)e0.w << 32);
+    const int od = (int)(o1 >> 56);
+    const u64 w1 = ((u64)depth << 56) | val, w0 = k2 ^ w1;
+    const uint4 ne = make_uint4((u32)w0, (u32)(w0 >> 32), (u32)w1, (u32)(w1 >> 32));
+    if (depth >= od) bkt[0] = ne;  // depth-preferred slot
+    else bkt[1] = ne;              // always-replace slot
+}
+
+struct TTLds {
+    Board b[kTTP];
+    u32 root[kTTP];
+    u32 sum[kTTP];
+    u64 k1[kTTP], k2[kTTP];
+    unsigned long long hit[kTTP];
+    u32 mv[kTCAP];
+    u32 wtot[kTWG / 64];
+    unsigned long long red[kTWG / 64];
+    u32 tile;
+    u32 nprobe, nhit;
+};
+
+struct LdsEmitT {
+    u32* list;
+    int pos;
+    int cap;
+    u32 tag;
+    __device__ __forceinline__ void operator()(u32 m) {
+        if (pos >= 0 && pos < cap) list[pos] = m | tag;
+        pos++;
+    }
+};
+
+template <class SL>
+__device__ void expand_tt_level(const SL& sl, const TTArgs& a, const ZTables* zt, TTLds& s) {
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const u32 ntiles = (a.n_in + kTTP - 1) / kTTP;
+    for (;;) {
+        if (tid == 0) {
+            s.tile = atomicAdd(a.tile_ctr, 1u);
+            s.nprobe = 0;
+            s.nhit = 0;
+        }
+        __syncthreads();
+        const u32 tile = s.tile;
+        if (tile >= ntiles) break;
+        const u32 p0 = tile * kTTP;
+        const u32 np = min((u32)kTTP, a.n_in - p0);
+        u32 cnt = 0;
+        if (tid < kTTP) {
+            if ((u32)tid < np) {
+                const Board b = a.in[p0 + tid];
+                s.b[tid] = b;
+                s.root[tid] = a.in_root[p0 + tid];
+                s.k1[tid] = a.in_k1[p0 + tid];
+                s.k2[tid] = a.in_k2[p0 + tid];
+                cnt = (u32)meta_count(b);
+            }
+            s.sum[tid] = 0;
+            s.hit[tid] = 0;
+        }
+        const u32 incl = t_wave_incl_scan(cnt, lane);
+        if (lane == 63) s.wtot[wid] = incl;
+        __syncthreads();
+        u32 woff = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < kTWG / 64; w++) {
+            const u32 t = s.wtot[w];
+            woff += (w < wid) ? t : 0;
+            tot += t;
+        }
+        const u32 pre = woff + incl - cnt;
+        for (u32 w0 = 0; w0 < tot; w0 += kTCAP) {
+            if (cnt && pre < w0 + kTCAP && pre + cnt > w0) {
+                const Board b = s.b[tid];
+                PosInfo pi;
+                analyse(b, sl, pi);
+                LdsEmitT e{s.mv, (int)pre - (int)w0, kTCAP, (u32)tid << 16};
+                generate(b, sl, pi, e);
+            }
+            __syncthreads();
+            const u32 m = min((u32)kTCAP, tot - w0);
+            // each wave walks 64 consecutive entries at a time (uniform loop)
+            for (u32 jb = (u32)wid * 64; jb < m; jb += kTWG) {
+                const u32 j = jb + lane;
+                const bool valid = j < m;
+                u32 par = 0, mvv = 0;
+                if (valid) {
+                    const u32 e = s.mv[j];
+                    par = (e >> 16) & (kTTP - 1);
+                    mvv = e & 0xFFFFu;
+                }
+                Board c = {};
+                u32 cc = 0;
+                bool keep = false;
+                u64 ck1 = 0, ck2 = 0;
+                if (valid) {
+                    const Board pb = s.b[par];
+                    c = make_move(pb, mvv);
+                    if (a.mode == kWrite) {
+                        const int black = (int)(s.root[par] >> 31);
+                        ck1 = zobrist_child(zt->k[0], s.k1[par], pb, mvv, black, c);
+                        ck2 = zobrist_child(zt->k[1], s.k2[par], pb, mvv, black, c);
+                        u64 v = 0;
+                        if (tt_probe(a.tt, a.tt_mask, ck1, ck2, a.child_depth, &v)) {
+                            atomicAdd(&s.hit[par], (unsigned long long)v);
+                        } else {
+                            keep = true;
+                        }
+                    }
+                    if (a.mode == kLeaf || keep) {
+                        cc = count_moves(c, sl);
+                        atomicAdd(&s.sum[par], cc);
+                    }
+                }
+                if (a.mode == kWrite) {
+                    const u64 bal = __ballot(keep);
+                    const u64 vbal = __ballot(valid);
+                    const u32 nk = (u32)__popcll(bal);
+                    if (lane == 0) {
+                        const u32 nv = (u32)__popcll(vbal);
+                        atomicAdd(&s.nprobe, nv);
+                        atomicAdd(&s.nhit, nv - nk);
+                    }
+                    u32 base = 0;
+                    if (lane == 0 && nk) base = atomicAdd(a.out_count, nk);
+                    base = __shfl(base, 0, 64);
+                    const u32 below = (u32)__popcll(bal & ((1ULL << lane) - 1));
+                    const u64 pos = (u64)base + below;
+                    if (keep) {
+                        if (pos < a.out_cap) {
+                            a.out[pos] = with_count(c, cc);
+                            a.out_root[pos] = s.root[par] ^ 0x80000000u;  // child's side to move
+                            a.out_k1[pos] = ck1;
+                            a.out_k2[pos] = ck2;
+                            a.out_parent[pos] = a.in_base + p0 + par;
+                            a.out_acc[pos] = 0;
+                        } else {
+                            atomicOr(a.err, 1u);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // accumulators: hits (write mode) or perft(parent, 2) (leaf mode)
+        unsigned long long tsum = 0;
+        if ((u32)tid < np) {
+            const unsigned long long add = (a.mode == kWrite) ? s.hit[tid] : (unsigned long long)s.sum[tid];
+            a.in_acc[p0 + tid] += add;
+            tsum = s.sum[tid];
+        }
+        if (a.total) {
+            const unsigned long long v = t_wave_sum64(tsum);
+            if (lane == 0) s.red[wid] = v;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            if (a.total) {
+                unsigned long long t = 0;
+                for (int w = 0; w < kTWG / 64; w++) t += s.red[w];
+                if (t) atomicAdd(a.total, t);
+            }
+            if (a.stats && s.nprobe) {
+                atomicAdd(&a.stats[0], (unsigned long long)s.nprobe);
+                atomicAdd(&a.stats[1], (unsigned long long)s.nhit);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int B>
+__global__ __launch_bounds__(kTWG) void k_expand_tt(const SliderTables* tabs, const ZTables* zt, TTArgs a) {
+    __shared__ TTLds s;
+    const auto sl = SlOfT<B>::make(tabs);
+    expand_tt_level(sl, a, zt, s);
+}
+
+// Roots: keys were computed on the host; probe at the full depth, count moves.
+template <int B>
+__global__ __launch_bounds__(256) void k_roots_tt(const SliderTables* tabs, Board* roots, const u64* k1,
+                                                  const u64* k2, u32 n, int depth, const uint4* tt, u64 mask,
+                                                  unsigned long long* acc, unsigned long long* total) {
+    const auto sl = SlOfT<B>::make(tabs);
+    const u32 i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const Board b = roots[i];
+    u64 v = 0;
+    if (tt && tt_probe(tt, mask, k1[i], k2[i], depth, &v)) {
+        acc[i] = v;
+        roots[i] = with_count(b, 0);  // answered: nothing to expand
+        return;
+    }
+    const u32 c = count_moves(b, sl);
+    roots[i] = with_count(b, c);
+    acc[i] = 0;
+    atomicAdd(total, (unsigned long long)c);
+}
+
+// Finished level: add each node's count to its parent and store it.
+__global__ __launch_bounds__(256) void k_upsweep(const u32* parent, const u64* k1, const u64* k2,
+                                                 const unsigned long long* acc, u32 n, unsigned long long* parent_acc,
+                                                 uint4* tt, u64 mask, int depth, unsigned long long* stores) {
+    const u32 i = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long st = 0;
+    if (i < n) {
+        const unsigned long long v = acc[i];
+        if (parent_acc) atomicAdd(&parent_acc[parent[i]], v);
+        if (tt) {
+            tt_store(tt, mask, k1[i], k2[i], depth, v);
+            st = 1;
+        }
+    }
+    if (stores) {
+        st = t_wave_sum64(st);
+        if ((threadIdx.x & 63) == 0 && st) atomicAdd(stores, st);
+    }
+}
+
+// ---- launchers ---------------------------------------------------------------
+static void* pick(int backend) {
+    switch (backend) {
+    case kKS: return reinterpret_cast<void*>(&k_expand_tt<kKS>);
+    case kPlain: return reinterpret_cast<void*>(&k_expand_tt<kPlain>);
+    default: return reinterpret_cast<void*>(&k_expand_tt<kFancy>);  // byte: global tables here
+    }
+}
+
+int expand_tt_grid(int backend, int n_cu) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pick(backend), kTWG, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return per_cu * n_cu;
+}
+
+hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables* zt, const TTArgs& a, int grid,
+                            hipStream_t s) {
+    if (!a.n_in) return hipSuccess;
+    const u32 tiles = (a.n_in + kTTP - 1) / kTTP;
+    if ((u32)grid > tiles) grid = (int)tiles;
+    TTArgs args = a;
+    void* kargs[] = {(void*)&tabs, (void*)&zt, (void*)&args};
+    return hipLaunchKernel(pick(backend), dim3(grid), dim3(kTWG), kargs, 0, s);
+}
+
+hipError_t launch_roots_tt(int backend, const SliderTables* tabs, Board* roots, const u64* k1, const u64* k2, u32 n,
+                           int depth, const uint4* tt, u64 mask, unsigned long long* acc, unsigned long long* total,
+                           hipStream_t s) {
+    if (!n) return hipSuccess;
+    const dim3 g((n + 255) / 256), b(256);
+    switch (backend) {
+    case kKS: hipLaunchKernelGGL(k_roots_tt<kKS>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total); break;
+    case kPlain: hipLaunchKernelGGL(k_roots_tt<kPlain>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total); break;
+    default: hipLaunchKernelGGL(k_roots_tt<kFancy>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_upsweep(const u32* parent, const u64* k1, const u64* k2, const unsigned long long* acc, u32 n,
+                          unsigned long long* parent_acc, uint4* tt, u64 mask, int depth, unsigned long long* stores,
+                          hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_upsweep, dim3((n + 255) / 256), dim3(256), 0, s, parent, k1, k2, acc, n, parent_acc, tt, mask,
+                       depth, stores);
+    return hipGetLastError();
+}
+
+}  // namespace pf

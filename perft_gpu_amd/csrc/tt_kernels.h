@@ -1,0 +1,52 @@
+// tt_kernels.h -- launchers of the transposition-table perft kernels (tt.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "backends_dev.h"
+#include "kernels.h"
+#include "tt.h"
+
+namespace pf {
+
+template <int B>
+using SlOfT = SlOf<B>;
+
+struct TTArgs {
+    // parents: a contiguous range of one level
+    const Board* in;
+    const u32* in_root;   // root index | side-to-move colour << 31
+    const u64* in_k1;
+    const u64* in_k2;
+    u32 n_in;
+    u32 in_base;          // index of in[0] within its level (parent indices)
+    unsigned long long* in_acc;  // parents' accumulators (+= hits or perft(.,2))
+    // children (write mode)
+    Board* out;
+    u32* out_root;
+    u64* out_k1;
+    u64* out_k2;
+    u32* out_parent;
+    unsigned long long* out_acc;
+    u64 out_cap;
+    u32* out_count;
+    unsigned long long* total;   // sum of the written children's move counts
+    uint4* tt;                   // 2 x 16-byte entries per bucket
+    u64 tt_mask;                 // buckets - 1
+    int child_depth;             // remaining depth of the children
+    int mode;                    // kWrite or kLeaf
+    u32* tile_ctr;
+    u32* err;
+    unsigned long long* stats;   // [0] probes, [1] hits
+};
+
+int expand_tt_grid(int backend, int n_cu);
+hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables* zt, const TTArgs& a, int grid,
+                            hipStream_t s);
+hipError_t launch_roots_tt(int backend, const SliderTables* tabs, Board* roots, const u64* k1, const u64* k2, u32 n,
+                           int depth, const uint4* tt, u64 mask, unsigned long long* acc, unsigned long long* total,
+                           hipStream_t s);
+hipError_t launch_upsweep(const u32* parent, const u64* k1, const u64* k2, const unsigned long long* acc, u32 n,
+                          unsigned long long* parent_acc, uint4* tt, u64 mask, int depth, unsigned long long* stores,
+                          hipStream_t s);
+
+}  // namespace pf

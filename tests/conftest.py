@@ -23,7 +23,7 @@ def build_harness():
     """Test-only build of the product's move generator for the CPU."""
     srcs = [HARNESS_SRC, os.path.join(ROOT, "perft_gpu_amd", "csrc", "tables.cpp")]
     deps = srcs + [os.path.join(ROOT, "perft_gpu_amd", "csrc", f)
-                   for f in ("board.h", "sliders.h", "movegen.h", "convert.h", "tables.h")]
+                   for f in ("board.h", "sliders.h", "movegen.h", "convert.h", "tables.h", "tt.h")]
     if os.path.exists(HARNESS) and all(os.path.getmtime(HARNESS) >= os.path.getmtime(d) for d in deps):
         return HARNESS
     os.makedirs(os.path.dirname(HARNESS), exist_ok=True)

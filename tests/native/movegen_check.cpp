@@ -8,6 +8,7 @@
 #include "../../perft_gpu_amd/csrc/convert.h"
 #include "../../perft_gpu_amd/csrc/movegen.h"
 #include "../../perft_gpu_amd/csrc/tables.h"
+#include "../../perft_gpu_amd/csrc/tt.h"
 
 using namespace pf;
 
@@ -103,5 +104,36 @@ int mg_children(const pf_pos48* p, pf_pos48* out) {
     generate(b, sl, pi, emit);
     for (int i = 0; i < n; i++) out[i] = to_pos48(make_move(b, mv[i]), !black);
     return n;
+}
+
+// full key of a position with key set `which` (0 bucket key, 1 verification key)
+uint64_t mg_zkey_full(const pf_pos48* p, uint64_t seed, int which) {
+    ZTables z;
+    zobrist_tables(z, seed);
+    return zobrist_full(z.k[which], to_board(*p), (int)(p->pawns >> 63));
+}
+
+// incremental child keys == full keys of the children, both key sets
+int mg_zkey_children_check(const pf_pos48* p, uint64_t seed) {
+    ZTables z;
+    zobrist_tables(z, seed);
+    const int black = (int)(p->pawns >> 63);
+    const Board b = to_board(*p);
+    const KSBackend sl{};
+    PosInfo pi;
+    analyse(b, sl, pi);
+    u32 mv[256];
+    int n = 0;
+    auto emit = [&](u32 m) { mv[n++] = m; };
+    generate(b, sl, pi, emit);
+    int bad = 0;
+    for (int w = 0; w < 2; w++) {
+        const u64 k = zobrist_full(z.k[w], b, black);
+        for (int i = 0; i < n; i++) {
+            const Board c = make_move(b, mv[i]);
+            if (zobrist_child(z.k[w], k, b, mv[i], black, c) != zobrist_full(z.k[w], c, !black)) bad++;
+        }
+    }
+    return bad;
 }
 }

@@ -161,3 +161,29 @@ def test_workload_positions_match_golden(pg):
         assert (t, d, s) == (want["t"], want["depth"], want["seed"])
         assert ["%016x" % int(x) for x in W.request_position(s)] == want["pos"]
     assert sum(g["c2"]["perft"]["5"]) == W.C2_LEAVES
+
+
+def test_zobrist_full_keys_match_oracle(pg, oracle, mg):
+    mg.mg_zkey_full.restype = ctypes.c_uint64
+    mg.mg_zkey_full.argtypes = [ctypes.POINTER(pg.Pos48), ctypes.c_uint64, ctypes.c_int]
+    salt = 0xD1B54A32D192ED03
+    for seed in (1, 0xC3C3):
+        t1, t2 = oracle.zobrist_table(seed), oracle.zobrist_table(seed ^ salt)
+        for row in pg.playouts(seed, 60, 1, 90):
+            p = pg.Pos48(*[int(x) for x in row])
+            o = oracle.from48(row)
+            assert mg.mg_zkey_full(ctypes.byref(p), seed, 0) == oracle.zobrist_key(o, t1)
+            assert mg.mg_zkey_full(ctypes.byref(p), seed, 1) == oracle.zobrist_key(o, t2)
+
+
+def test_zobrist_incremental_keys(pg, mg):
+    """the device's incremental key update equals a full recomputation"""
+    mg.mg_zkey_children_check.argtypes = [ctypes.POINTER(pg.Pos48), ctypes.c_uint64]
+    rows = list(pg.playouts(4, 300, 1, 110))
+    rows += [tuple(pg.fen_to_pos(f).as_tuple()) for f in
+             ["r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq - 0 1",
+              "r3k2r/Pppp1ppp/1b3nbN/nP6/BBP1P3/q4N2/Pp1P2PP/R2Q1RK1 w kq - 0 1",
+              "8/8/1k6/2b5/2pP4/8/5K2/8 b - d3 0 1", "r3k2r/1P6/8/8/8/8/6p1/R3K2R b KQkq - 0 1"]]
+    for row in rows:
+        p = pg.Pos48(*[int(x) for x in row])
+        assert mg.mg_zkey_children_check(ctypes.byref(p), 99) == 0, pg.pos_to_fen(p)
