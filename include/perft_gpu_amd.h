@@ -130,6 +130,22 @@ pf_status pf_get_stats(pf_ctx *ctx, pf_stats *out);
 pf_status pf_sync(pf_ctx *ctx);
 pf_status pf_reset_stats(pf_ctx *ctx);
 
+/* Serving (no reference counterpart; the reference answers one perft per
+ * run): a background loop admits submitted requests while one of max_jobs
+ * slots -- each an equal share of the frontier arena -- is free, advances all
+ * admitted requests together (one captured-graph step per breadth-first
+ * level), shares the transposition table between them and reports each
+ * result with its arrival and completion times.  While serving, pf_perft*
+ * and pf_divide on the same context return PF_EBUSY. */
+pf_status pf_serve_start(pf_ctx *ctx, int max_jobs, uint64_t job_bytes);
+pf_status pf_submit(pf_ctx *ctx, const pf_pos48 *pos, int depth, uint64_t tag);
+/* Copies up to max finished results; returns how many (>= 0) or a pf_status. */
+int pf_poll(pf_ctx *ctx, pf_result *out, int max);
+/* Finishes every admitted and queued request, then stops the loop. */
+pf_status pf_serve_stop(pf_ctx *ctx);
+/* Seconds on the library clock used for pf_result times. */
+double pf_now(void);
+
 /* Diagnostics (no reference counterpart): measure a ceiling on this device.
  * kind 0: int32 VALU lane-ops/s; 1: streaming copy, read+write bytes/s over a
  * `bytes` buffer; 2: random 16-byte read-modify-writes/s over a `bytes` table. */

@@ -69,7 +69,7 @@ _lib = None
 EXPORTS = ["pf_init", "pf_destroy", "pf_strerror", "pf_fen_to_pos", "pf_pos_to_fen", "pf_perft",
            "pf_perft_batch", "pf_perft_batch_mode", "pf_divide", "pf_move_to_uci", "pf_apply_uci",
            "pf_set_backend", "pf_get_stats", "pf_reset_stats", "pf_sync", "pf_probe", "pf_playouts",
-           "pf_count_moves_host"]
+           "pf_count_moves_host", "pf_serve_start", "pf_submit", "pf_poll", "pf_serve_stop", "pf_now"]
 
 
 def load_library(path=LIB_PATH):
@@ -250,6 +250,28 @@ class Engine:
         out = ctypes.c_double()
         _check(self._lib.pf_probe(self._ctx, kind, nbytes, iters, ctypes.byref(out)), "probe")
         return out.value
+
+    # ---- serving (continuous batching) ----
+    def serve_start(self, max_jobs=64, job_bytes=0):
+        _check(self._lib.pf_serve_start(self._ctx, max_jobs, job_bytes), "serve_start")
+
+    def submit(self, pos, depth, tag):
+        p = pos if isinstance(pos, Pos48) else Pos48(*[int(x) for x in pos])
+        _check(self._lib.pf_submit(self._ctx, ctypes.byref(p), depth, tag), "submit")
+
+    def poll(self, max_results=4096):
+        buf = (Result * max_results)()
+        n = self._lib.pf_poll(self._ctx, buf, max_results)
+        if n < 0:
+            raise PerftError(n, "poll")
+        return [{"tag": buf[i].tag, "nodes": buf[i].nodes, "status": buf[i].status, "depth": buf[i].depth,
+                 "submit_s": buf[i].submit_s, "done_s": buf[i].done_s} for i in range(n)]
+
+    def serve_stop(self):
+        _check(self._lib.pf_serve_stop(self._ctx), "serve_stop")
+
+    def now(self):
+        return self._lib.pf_now()
 
     def reset_stats(self):
         _check(self._lib.pf_reset_stats(self._ctx), "reset_stats")

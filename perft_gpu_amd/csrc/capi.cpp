@@ -7,9 +7,11 @@
 
 #include "../../include/perft_gpu_amd.h"
 #include "engine.h"
+#include "serve.h"
 
 struct pf_ctx {
     pf::Engine eng;
+    pf::Server* srv = nullptr;
 };
 
 using namespace pf;
@@ -84,6 +86,11 @@ pf_status pf_init(const pf_config* cfg, pf_ctx** out) {
 
 void pf_destroy(pf_ctx* ctx) {
     if (!ctx) return;
+    if (ctx->srv) {
+        ctx->srv->stop();
+        delete ctx->srv;
+        ctx->srv = nullptr;
+    }
     ctx->eng.destroy();
     delete ctx;
 }
@@ -181,12 +188,14 @@ static pf_status check_depth(int depth) { return (depth < 0 || depth > 16) ? PF_
 pf_status pf_perft(pf_ctx* ctx, const pf_pos48* pos, int depth, uint64_t* out) {
     if (!ctx || !pos || !out || check_depth(depth)) return PF_EINVAL;
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    if (ctx->eng.serving) return PF_EBUSY;
     return ctx->eng.perft_levels(pos, 1, depth, out);
 }
 
 pf_status pf_perft_batch_mode(pf_ctx* ctx, const pf_pos48* pos, int n, int depth, int mode, uint64_t* out) {
     if (!ctx || (n > 0 && (!pos || !out)) || n < 0 || check_depth(depth)) return PF_EINVAL;
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    if (ctx->eng.serving) return PF_EBUSY;
     if (mode == PF_MODE_LEVELS) return ctx->eng.perft_levels(pos, n, depth, out);
     pf_status s = ctx->eng.perft_persistent(pos, n, depth, out);
     if (s == PF_ENOMEM && mode == PF_MODE_AUTO) s = ctx->eng.perft_levels(pos, n, depth, out);
@@ -256,6 +265,48 @@ pf_status pf_get_stats(pf_ctx* ctx, pf_stats* out) {
     *out = ctx->eng.st;
     return PF_OK;
 }
+
+pf_status pf_serve_start(pf_ctx* ctx, int max_jobs, uint64_t job_bytes) {
+    if (!ctx || max_jobs < 1) return PF_EINVAL;
+    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    if (ctx->srv) return PF_EBUSY;
+    if (ctx->eng.arena_top != 0) return PF_EBUSY;
+    ctx->srv = new (std::nothrow) Server(ctx->eng, max_jobs, job_bytes);
+    if (!ctx->srv) return PF_ENOMEM;
+    const pf_status s = ctx->srv->start();
+    if (s != PF_OK) {
+        delete ctx->srv;
+        ctx->srv = nullptr;
+        return s;
+    }
+    ctx->eng.serving = true;
+    return PF_OK;
+}
+
+pf_status pf_submit(pf_ctx* ctx, const pf_pos48* pos, int depth, uint64_t tag) {
+    if (!ctx || !pos) return PF_EINVAL;
+    if (!ctx->srv) return PF_EINVAL;
+    return ctx->srv->submit(*pos, depth, tag);
+}
+
+int pf_poll(pf_ctx* ctx, pf_result* out, int max) {
+    if (!ctx || !out || max < 0) return PF_EINVAL;
+    if (!ctx->srv) return PF_EINVAL;
+    return ctx->srv->poll(out, max);
+}
+
+pf_status pf_serve_stop(pf_ctx* ctx) {
+    if (!ctx) return PF_EINVAL;
+    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    if (!ctx->srv) return PF_OK;
+    ctx->srv->stop();
+    delete ctx->srv;
+    ctx->srv = nullptr;
+    ctx->eng.serving = false;
+    return PF_OK;
+}
+
+double pf_now(void) { return now_s(); }
 
 pf_status pf_sync(pf_ctx* ctx) {
     if (!ctx) return PF_EINVAL;

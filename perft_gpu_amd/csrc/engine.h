@@ -43,6 +43,7 @@ struct Engine {
     bool timing = true;
 
     int depth = 0;  // depth of the current run
+    bool serving = false;  // the serving loop owns the arena
 
     // transposition table (tt.h): 2^k buckets of two 16-byte entries
     uint4* d_tt = nullptr;

@@ -1,0 +1,82 @@
+// serve.h -- continuous-batching perft server.
+#pragma once
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "engine.h"
+
+namespace pf {
+
+double now_s();  // library clock, seconds
+
+class Server {
+  public:
+    Server(Engine& e, int max_jobs, u64 job_bytes);
+    ~Server();
+    pf_status start();
+    void stop();
+    pf_status submit(const pf_pos48& pos, int depth, u64 tag);
+    int poll(pf_result* out, int max);
+    void counters(u64* steps, u64* segs, double* busy_s);
+
+  private:
+    struct Frame {
+        Engine::TLevel lv;
+        u64 off, m, n_next, end;  // parents [off, off+m) of [.., end) when chunking
+        int L;
+        int state;
+        bool chunking;
+        Engine::TLevel nx;
+        u64 cap, mark, written;
+    };
+    struct Job {
+        u64 tag = 0;
+        int depth = 0;
+        double t_submit = 0;
+        int slot = -1;
+        pf_pos48 pos{};
+        u64 top = 0;  // stack top within the job's arena region
+        std::vector<Frame> st;
+        int pending = 0;  // 0 none, 1 root, 2 expand, 3 upsweep
+        int seg = -1;
+        bool done = false;
+    };
+
+    void run();
+    bool plan(Job& j);
+    void finish(Job& j, u64 nodes, int status);
+    bool alloc_level(Job& j, u64 n, Engine::TLevel* lv);
+    pf_status build_graph();
+
+    Engine& e_;
+    int max_jobs_;
+    u64 job_bytes_;
+    std::thread th_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    bool stop_ = false, running_ = false;
+    std::deque<Job> queue_;        // submitted, not yet admitted
+    std::vector<pf_result> done_;  // finished, not yet polled
+    std::vector<Job> jobs_;        // indexed by slot
+    std::vector<int> free_slots_;
+    // device / pinned resources
+    StepTable* h_tab_ = nullptr;
+    StepTable* d_tab_ = nullptr;
+    Board* root_b_ = nullptr;
+    u32* root_w_ = nullptr;
+    u64* root_k1_ = nullptr;
+    u64* root_k2_ = nullptr;
+    u32* root_p_ = nullptr;
+    unsigned long long* slot_acc_ = nullptr;
+    unsigned long long* h_acc_ = nullptr;
+    int grid_ = 0, up_grid_ = 0;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    u64 steps_ = 0, segs_ = 0;
+    double busy_ = 0;
+};
+
+}  // namespace pf

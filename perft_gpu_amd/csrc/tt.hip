@@ -87,19 +87,15 @@ struct LdsEmitT {
     }
 };
 
+// One tile (kTTP parents) of one level.  Every thread of the block calls it.
 template <class SL>
-__device__ void expand_tt_level(const SL& sl, const TTArgs& a, const ZTables* zt, TTLds& s) {
+__device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTables* zt, TTLds& s) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const u32 ntiles = (a.n_in + kTTP - 1) / kTTP;
-    for (;;) {
+    {
         if (tid == 0) {
-            s.tile = atomicAdd(a.tile_ctr, 1u);
             s.nprobe = 0;
             s.nhit = 0;
         }
-        __syncthreads();
-        const u32 tile = s.tile;
-        if (tile >= ntiles) break;
         const u32 p0 = tile * kTTP;
         const u32 np = min((u32)kTTP, a.n_in - p0);
         u32 cnt = 0;
@@ -108,8 +104,10 @@ __device__ void expand_tt_level(const SL& sl, const TTArgs& a, const ZTables* zt
                 const Board b = a.in[p0 + tid];
                 s.b[tid] = b;
                 s.root[tid] = a.in_root[p0 + tid];
-                s.k1[tid] = a.in_k1[p0 + tid];
-                s.k2[tid] = a.in_k2[p0 + tid];
+                if (a.tt) {
+                    s.k1[tid] = a.in_k1[p0 + tid];
+                    s.k2[tid] = a.in_k2[p0 + tid];
+                }
                 cnt = (u32)meta_count(b);
             }
             s.sum[tid] = 0;
@@ -154,11 +152,13 @@ __device__ void expand_tt_level(const SL& sl, const TTArgs& a, const ZTables* zt
                     const Board pb = s.b[par];
                     c = make_move(pb, mvv);
                     if (a.mode == kWrite) {
-                        const int black = (int)(s.root[par] >> 31);
-                        ck1 = zobrist_child(zt->k[0], s.k1[par], pb, mvv, black, c);
-                        ck2 = zobrist_child(zt->k[1], s.k2[par], pb, mvv, black, c);
                         u64 v = 0;
-                        if (tt_probe(a.tt, a.tt_mask, ck1, ck2, a.child_depth, &v)) {
+                        if (a.tt) {
+                            const int black = (int)(s.root[par] >> 31);
+                            ck1 = zobrist_child(zt->k[0], s.k1[par], pb, mvv, black, c);
+                            ck2 = zobrist_child(zt->k[1], s.k2[par], pb, mvv, black, c);
+                        }
+                        if (a.tt && tt_probe(a.tt, a.tt_mask, ck1, ck2, a.child_depth, &v)) {
                             atomicAdd(&s.hit[par], (unsigned long long)v);
                         } else {
                             keep = true;
@@ -187,8 +187,10 @@ __device__ void expand_tt_level(const SL& sl, const TTArgs& a, const ZTables* zt
                         if (pos < a.out_cap) {
                             a.out[pos] = with_count(c, cc);
                             a.out_root[pos] = s.root[par] ^ 0x80000000u;  // child's side to move
-                            a.out_k1[pos] = ck1;
-                            a.out_k2[pos] = ck2;
+                            if (a.tt) {
+                                a.out_k1[pos] = ck1;
+                                a.out_k2[pos] = ck2;
+                            }
                             a.out_parent[pos] = a.in_base + p0 + par;
                             a.out_acc[pos] = 0;
                         } else {
@@ -223,6 +225,19 @@ __device__ void expand_tt_level(const SL& sl, const TTArgs& a, const ZTables* zt
             }
         }
         __syncthreads();
+    }
+}
+
+template <class SL>
+__device__ void expand_tt_level(const SL& sl, const TTArgs& a, const ZTables* zt, TTLds& s) {
+    const u32 ntiles = (a.n_in + kTTP - 1) / kTTP;
+    for (;;) {
+        if (threadIdx.x == 0) s.tile = atomicAdd(a.tile_ctr, 1u);
+        __syncthreads();
+        const u32 tile = s.tile;
+        __syncthreads();
+        if (tile >= ntiles) break;
+        tt_tile(sl, a, tile, zt, s);
     }
 }
 
@@ -272,6 +287,115 @@ __global__ __launch_bounds__(256) void k_upsweep(const u32* parent, const u64* k
         st = t_wave_sum64(st);
         if ((threadIdx.x & 63) == 0 && st) atomicAdd(stores, st);
     }
+}
+
+// ---- serving step: many jobs' segments in one launch ---------------------------
+// Tiles of all expand segments form one queue (StepTable::tile_prefix); a
+// block copies its segment's arguments to LDS and runs the tile.  Segments of
+// different jobs touch disjoint memory except the lockless table.
+template <int B>
+__global__ __launch_bounds__(kTWG) void k_step(const SliderTables* tabs, const ZTables* zt, StepTable* t) {
+    __shared__ TTLds s;
+    __shared__ TTArgs sa;
+    __shared__ u32 s_tile, s_seg;
+    const auto sl = SlOfT<B>::make(tabs);
+    const u32 nseg = t->nseg, total = t->tile_prefix[nseg];
+    u32 cur = 0xFFFFFFFFu;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const u32 g = atomicAdd(&t->tile_ctr, 1u);
+            u32 seg = 0;
+            if (g < total) {
+                u32 lo = 0, hi = nseg;  // largest seg with tile_prefix[seg] <= g
+                while (hi - lo > 1) {
+                    const u32 mid = (lo + hi) >> 1;
+                    if (t->tile_prefix[mid] <= g) lo = mid; else hi = mid;
+                }
+                seg = lo;
+            }
+            s_tile = g;
+            s_seg = seg;
+        }
+        __syncthreads();
+        const u32 g = s_tile, seg = s_seg;
+        if (g >= total) break;
+        if (seg != cur) {
+            if (threadIdx.x == 0) sa = t->seg[seg];
+            cur = seg;
+        }
+        __syncthreads();
+        tt_tile(sl, sa, g - t->tile_prefix[seg], zt, s);
+    }
+}
+
+// Upsweep segments: thread i of the concatenation finds its segment by search.
+__global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 mask, unsigned long long* stores) {
+    const u32 nup = t->nup, total = t->up_prefix[nup];
+    for (u32 i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256) {
+        u32 lo = 0, hi = nup;
+        while (hi - lo > 1) {
+            const u32 mid = (lo + hi) >> 1;
+            if (t->up_prefix[mid] <= i) lo = mid; else hi = mid;
+        }
+        const UpSeg& u = t->up[lo];
+        const u32 k = i - t->up_prefix[lo];
+        const unsigned long long v = u.acc[k];
+        if (u.parent_acc) atomicAdd(&u.parent_acc[u.parent[k]], v);
+        if (tt) tt_store(tt, mask, u.k1[k], u.k2[k], u.depth, v);
+    }
+    (void)stores;
+}
+
+// New requests: probe the table at full depth, count the root's moves.
+template <int B>
+__global__ __launch_bounds__(64) void k_roots_step(const SliderTables* tabs, StepTable* t, const uint4* tt, u64 mask) {
+    const auto sl = SlOfT<B>::make(tabs);
+    const u32 i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= t->nroot) return;  // roots are few: a 64-thread block each 64
+    const RootSeg& r = t->root[i];
+    const Board b = r.b;
+    *r.dst_w = r.word;
+    if (tt) {
+        *r.dst_k1 = r.k1;
+        *r.dst_k2 = r.k2;
+    }
+    u64 v = 0;
+    if (tt && r.depth >= 2 && tt_probe(tt, mask, r.k1, r.k2, r.depth, &v)) {
+        *r.acc = v;
+        *r.dst_b = with_count(b, 0);
+        t->rout[i].hit = 1;
+        t->rout[i].count = 0;
+        return;
+    }
+    const u32 c = count_moves(b, sl);
+    *r.dst_b = with_count(b, c);
+    *r.acc = (r.depth == 1) ? c : 0;
+    t->rout[i].hit = 0;
+    t->rout[i].count = c;
+}
+
+int step_grid(int backend, int n_cu) {
+    const void* f = backend == kKS ? (const void*)&k_step<kKS> : backend == kPlain ? (const void*)&k_step<kPlain>
+                                                                                   : (const void*)&k_step<kFancy>;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kTWG, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    return per_cu * n_cu;
+}
+
+hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
+                       int grid, int up_grid, hipStream_t s) {
+    switch (backend) {
+    case kKS: hipLaunchKernelGGL(k_roots_step<kKS>, dim3(kMaxRoots / 64), dim3(64), 0, s, tabs, t, tt, mask); break;
+    case kPlain: hipLaunchKernelGGL(k_roots_step<kPlain>, dim3(kMaxRoots / 64), dim3(64), 0, s, tabs, t, tt, mask); break;
+    default: hipLaunchKernelGGL(k_roots_step<kFancy>, dim3(kMaxRoots / 64), dim3(64), 0, s, tabs, t, tt, mask); break;
+    }
+    switch (backend) {
+    case kKS: hipLaunchKernelGGL(k_step<kKS>, dim3(grid), dim3(kTWG), 0, s, tabs, zt, t); break;
+    case kPlain: hipLaunchKernelGGL(k_step<kPlain>, dim3(grid), dim3(kTWG), 0, s, tabs, zt, t); break;
+    default: hipLaunchKernelGGL(k_step<kFancy>, dim3(grid), dim3(kTWG), 0, s, tabs, zt, t); break;
+    }
+    hipLaunchKernelGGL(k_up_step, dim3(up_grid), dim3(256), 0, s, t, tt, mask, nullptr);
+    return hipGetLastError();
 }
 
 // ---- launchers ---------------------------------------------------------------

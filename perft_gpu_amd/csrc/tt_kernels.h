@@ -39,6 +39,56 @@ struct TTArgs {
     unsigned long long* stats;   // [0] probes, [1] hits
 };
 
+// One serving step (serve.cpp): expand/leaf segments of many jobs, their
+// upsweep segments and newly admitted roots, described in device memory so
+// that the step can be replayed from a captured graph.
+constexpr int kMaxSegs = 512;
+constexpr int kMaxUps = 512;
+constexpr int kMaxRoots = 256;
+
+struct UpSeg {
+    const u32* parent;
+    const u64* k1;
+    const u64* k2;
+    const unsigned long long* acc;
+    unsigned long long* parent_acc;
+    u32 n;
+    int depth;
+};
+struct RootSeg {  // payload travels in the table; the kernel scatters it
+    Board b;
+    u64 k1, k2;
+    u32 word;  // slot | side-to-move colour << 31
+    int depth;
+    Board* dst_b;
+    u64* dst_k1;
+    u64* dst_k2;
+    u32* dst_w;
+    unsigned long long* acc;
+};
+struct SegOut {
+    u32 out_count;
+    u32 err;
+    unsigned long long total;
+};
+struct RootOut {
+    u32 hit;
+    u32 count;
+};
+struct StepTable {
+    u32 nseg, nup, nroot, tile_ctr;
+    u32 tile_prefix[kMaxSegs + 1];
+    u32 up_prefix[kMaxUps + 1];
+    TTArgs seg[kMaxSegs];
+    UpSeg up[kMaxUps];
+    RootSeg root[kMaxRoots];
+    SegOut out[kMaxSegs];
+    RootOut rout[kMaxRoots];
+};
+
+int step_grid(int backend, int n_cu);
+hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
+                       int grid, int up_grid, hipStream_t s);
 int expand_tt_grid(int backend, int n_cu);
 hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables* zt, const TTArgs& a, int grid,
                             hipStream_t s);
