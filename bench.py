@@ -1,33 +1,38 @@
 #!/usr/bin/env python3
-"""Benchmark: leaf nodes/sec of batched perft on one MI355X.
+"""Benchmark: leaf nodes/sec of batched and served perft on one MI355X.
 
-Workload (one "step"): the C2 batch -- 256 seeded random-playout positions
-(8..40 plies, mate/stalemate discarded), perft depth 5 each, TT off, in one
-persistent cooperative launch (pf_perft_batch_mode(..., PF_MODE_PERSISTENT)).
-The positions live in HBM only for the duration of a call, so every step
-uploads 256 x 48 bytes; that upload is inside the timed region.
+Headline ("value"): the C2 batch -- 256 seeded random-playout positions
+(8..40 plies, mate/stalemate discarded), perft depth 5 each, TT off -- in one
+persistent cooperative launch per step (pf_perft_batch_mode(..., PERSISTENT)).
+A step includes the 256 x 48-byte upload of the positions.
 
-Printed JSON line (rank 0 only):
-  value         leaf nodes/sec of the batched path over the timed steps
+Also reported on the same box, outside the timed region:
   roofline      integer-VALU roofline of the persistent kernel: algorithmic
-                int32 lane-ops per launch (op model x leaves) / launch time,
-                against 256 CU x 64 lanes x 2.4 GHz = 39.3 T lane-op/s
-  plain         the same positions through the plain per-position path
-  backends      batched nodes/sec per slider backend
-  serving       p50/p99 request latency + nodes/sec of the serving loop
+                int32 lane-ops per launch (C2 op model x leaves) / launch time
+                (HIP events on the engine's stream) against MI355X's published
+                vector rate, 157.3 TFLOP/s FP32 = 78.6 T lane-op/s
+  plain         the same 256 positions through the plain per-position path
+  backends      batched nodes/sec per slider backend, and the level mode
+  serving       C4: Poisson arrivals (50 req/s) replayed in real time for
+                --serve-seconds, depths {3:.30,4:.30,5:.25,6:.12,7:.03},
+                32 GiB shared TT: p50/p99 latency and nodes/sec
+  c3            one 20-30-ply middlegame position, perft 8, 64 GiB TT
+  c5            the first --c5-positions of the 4096 high-mobility positions,
+                perft 6, 128 GiB TT
   cpu_baseline  the oracle (CPU restatement of the reference perft, fancy
-                magics) on the host cores, same positions, same depth
+                magics) on the host cores, same C2 positions, same depth
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # int32 lane-ops/s, nominal MI355X
+VALU_PEAK_TOPS = 157.3 / 2  # published FP32 vector TFLOP/s / 2 flops per lane-op
 
 
 def log(*a):
@@ -43,7 +48,6 @@ def cpu_baseline(positions, depth, gpu_counts, budget_s=20.0):
     orc = Oracle()
     orc.set_backend(2)
     threads = max(1, min(os.cpu_count() or 1, 16))
-    # size the sample to ~budget_s of CPU work from a quick single-thread probe
     t = time.perf_counter()
     probe = orc.perft_batch(positions[:4], depth - 1, nthreads=1)
     dt = max(time.perf_counter() - t, 1e-6)
@@ -54,11 +58,97 @@ def cpu_baseline(positions, depth, gpu_counts, budget_s=20.0):
     t = time.perf_counter()
     got = orc.perft_batch(positions[:n], depth, nthreads=threads)
     el = time.perf_counter() - t
-    match = bool((got == gpu_counts[:n]).all())
     return {"value": float(got.sum()) / el, "unit": "nodes/s", "cores": threads, "kind": "port",
-            "sample": "first %d of the %d positions, perft(%d) each, oracle fancy magics, %d threads, %.2f s"
+            "sample": "first %d of the %d C2 positions, perft(%d) each, oracle fancy magics, %d threads, %.2f s"
                       % (n, len(positions), depth, threads, el),
-            "counts_match_gpu": match}
+            "counts_match_gpu": bool((got == gpu_counts[:n]).all())}
+
+
+def run_serving(pg, W, seconds, backend, paced=True):
+    """Replay the C4 stream's first `seconds` of arrivals; paced=False submits
+    everything at once (capacity)."""
+    cfg = W.C4
+    reqs = W.serving_requests(cfg)
+    reqs = [r for r in reqs if r[0] < seconds]
+    poss = [W.request_position(s) for _, _, s in reqs]
+    eng = pg.Engine(backend=backend, arena_bytes=96 << 30, tt_bytes=cfg["tt_bytes"], zobrist_seed=cfg["seed"])
+    eng.serve_start(max_jobs=128)
+    results = {}
+    done = threading.Event()
+
+    def poller():
+        while not done.is_set() or len(results) < len(reqs):
+            for r in eng.poll():
+                results[r["tag"]] = r
+            if len(results) >= len(reqs):
+                break
+            time.sleep(0.0002)
+
+    th = threading.Thread(target=poller)
+    th.start()
+    t0 = time.perf_counter()
+    c0 = eng.now()
+    for i, ((t, d, _), p) in enumerate(zip(reqs, poss)):
+        wait = t - (time.perf_counter() - t0)
+        if paced and wait > 0:
+            time.sleep(wait)
+        eng.submit(p, d, i)
+    done.set()
+    th.join(timeout=600)
+    eng.serve_stop()
+    eng.close()
+    lat = sorted(r["done_s"] - r["submit_s"] for r in results.values())
+    nodes = sum(r["nodes"] for r in results.values())
+    span = max(r["done_s"] for r in results.values()) - c0
+    bad = sum(1 for r in results.values() if r["status"] != 0)
+    by_depth = {}
+    for r in results.values():
+        by_depth.setdefault(r["depth"], []).append(r["done_s"] - r["submit_s"])
+
+    def pct(v, q):
+        return v[min(len(v) - 1, int(q * len(v)))] * 1e3 if v else None
+
+    return {"paced": paced, "requests": len(reqs), "completed": len(results), "errors": bad,
+            "p50_ms": pct(lat, 0.50), "p99_ms": pct(lat, 0.99), "max_ms": lat[-1] * 1e3 if lat else None,
+            "nodes": nodes, "nodes_per_s": nodes / span if span > 0 else None, "span_s": span,
+            "arrival_rate_per_s": cfg["rate"], "seconds": seconds,
+            "p50_ms_by_depth": {str(d): pct(sorted(v), 0.5) for d, v in sorted(by_depth.items())},
+            "p99_ms_by_depth": {str(d): pct(sorted(v), 0.99) for d, v in sorted(by_depth.items())}}
+
+
+def run_plain_requests(pg, W, seconds, backend):
+    """The same request stream through plain per-position perft, one request
+    after another (same shared-TT configuration)."""
+    cfg = W.C4
+    reqs = [r for r in W.serving_requests(cfg) if r[0] < seconds]
+    eng = pg.Engine(backend=backend, arena_bytes=96 << 30, tt_bytes=cfg["tt_bytes"], zobrist_seed=cfg["seed"])
+    try:
+        nodes = 0
+        t = time.perf_counter()
+        for _, d, s in reqs:
+            nodes += eng.perft(pg.Pos48(*[int(x) for x in W.request_position(s)]), d)
+        el = time.perf_counter() - t
+        return {"requests": len(reqs), "nodes": nodes, "seconds": el, "nodes_per_s": nodes / el}
+    finally:
+        eng.close()
+
+
+def run_tt_workload(pg, W, cfg, positions, depth, backend, arena, check=None):
+    eng = pg.Engine(backend=backend, arena_bytes=arena, tt_bytes=cfg["tt_bytes"], zobrist_seed=cfg["seed"])
+    try:
+        t = time.perf_counter()
+        got = eng.perft_batch(positions, depth)
+        el = time.perf_counter() - t
+        st = eng.stats()
+        out = {"depth": depth, "positions": len(positions), "nodes": int(got.sum()), "seconds": el,
+               "nodes_per_s": float(got.sum()) / el, "tt_gib": cfg["tt_bytes"] >> 30,
+               "tt_probes": st["tt_probes"], "tt_hits": st["tt_hits"], "tt_stores": st["tt_stores"],
+               "chunks": st["chunks"], "peak_arena_gib": st["peak_arena_bytes"] / 2 ** 30}
+        if check is not None:
+            out["counts_match"] = bool(check(got))
+        return out
+    finally:
+        eng.close()
 
 
 def main():
@@ -67,7 +157,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--backend", default="ks")
-    ap.add_argument("--no-extras", action="store_true", help="skip plain/backend/serving/cpu comparisons")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--c5-positions", type=int, default=256)
+    ap.add_argument("--no-extras", action="store_true", help="skip every comparison run")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -116,15 +208,18 @@ def main():
         "dtype": "u64",
         "data": "synthetic: seeded random legal playouts from the initial position",
         "config": {"workload": cfg["name"], "positions": len(pos), "depth": depth, "tt": "off",
-                   "backend": args.backend, "mode": "persistent", "leaves_per_step": leaves},
+                   "backend": args.backend, "mode": "persistent", "leaves_per_step": leaves,
+                   "counts_match_golden": leaves == W.C2_LEAVES},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
-                     "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
+                     "frac": achieved / VALU_PEAK_TOPS,
+                     # HBM bytes per launch: FETCH_SIZE x2.00 + WRITE_SIZE x1.00 (rocprofv3 --pmc, calibrated
+                     # on a copy of known size; profiles/k_persistent_ks_c2_d5_counters.md)
+                     "traffic": 9.105e8,
                      "kernel": "k_persistent<%s>" % args.backend, "kernel_ms": kernel_ms,
                      "ops_per_leaf": W.C2_OPS_PER_LEAF},
     }
 
     if not args.no_extras:
-        # plain per-position perft: the same positions one at a time
         eng.reset_stats()
         t = time.perf_counter()
         plain = np.array([eng.perft(pg.Pos48(*[int(x) for x in r]), depth) for r in pos], dtype=np.uint64)
@@ -133,7 +228,6 @@ def main():
                          "counts_match_batched": bool((plain == counts).all())}
         line["batched_over_plain"] = value / (leaves / tp)
         log("plain: %.3e nodes/s  ratio %.2f" % (leaves / tp, line["batched_over_plain"]))
-        # backends and the host-driven level mode
         be = {}
         for b in ("ks", "plain", "fancy", "byte"):
             eng.set_backend(b)
@@ -150,10 +244,38 @@ def main():
         be["levels_mode_" + args.backend] = {"value": leaves / (time.perf_counter() - t),
                                              "counts_match": bool((c == counts).all())}
         line["backends"] = be
+    eng.close()
+
+    if not args.no_extras:
+        try:
+            line["serving"] = run_serving(pg, W, args.serve_seconds, args.backend)
+            log("serving: %s" % line["serving"])
+            cap = run_serving(pg, W, args.serve_seconds, args.backend, paced=False)
+            line["serving_capacity"] = cap
+            pl = run_plain_requests(pg, W, args.serve_seconds, args.backend)
+            line["plain_same_requests"] = pl
+            line["served_over_plain"] = cap["nodes_per_s"] / pl["nodes_per_s"]
+            log("serving capacity %.3e nodes/s vs plain %.3e" % (cap["nodes_per_s"], pl["nodes_per_s"]))
+        except Exception as e:  # report, do not lose the headline
+            line["serving_error"] = repr(e)
+        try:
+            import json as _j
+            with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
+                g = _j.load(f)
+            c3 = W.positions(W.C3)
+            line["c3"] = run_tt_workload(pg, W, W.C3, c3, 7, args.backend, 128 << 30,
+                                         check=lambda got: int(got[0]) == g["c3"]["perft"]["7"][0])
+            line["c3"]["d8"] = run_tt_workload(pg, W, W.C3, c3, 8, args.backend, 128 << 30)
+            log("c3: %s" % line["c3"])
+            c5 = W.positions(W.C5)[:args.c5_positions]
+            line["c5"] = run_tt_workload(pg, W, W.C5, c5, 6, args.backend, 96 << 30,
+                                         check=lambda got: [int(x) for x in got[:8]] == g["c5"]["perft_first8_d6"])
+            log("c5: %s" % line["c5"])
+        except Exception as e:
+            line["tt_runs_error"] = repr(e)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(pos, depth, counts)
         log("cpu: %s" % line["cpu_baseline"])
-    eng.close()
     print(json.dumps(line), flush=True)
 
 

@@ -53,9 +53,10 @@ typedef enum pf_backend {
 } pf_backend;
 
 typedef enum pf_mode {
-    PF_MODE_AUTO = 0,        /* persistent single launch when it fits, else levels */
+    PF_MODE_AUTO = 0,        /* TT configured: levels with the table; else persistent
+                                single launch when it fits, else levels */
     PF_MODE_LEVELS = 1,      /* host-driven level loop with depth-first chunking */
-    PF_MODE_PERSISTENT = 2   /* one cooperative launch for the whole batch */
+    PF_MODE_PERSISTENT = 2   /* one cooperative launch for the whole batch (no TT) */
 } pf_mode;
 
 typedef struct pf_config {

@@ -172,7 +172,10 @@ static u64 T_fancy_tab[FSM_TABLE_ENTRIES];
 static u8 T_byte_tab[FSM_TABLE_ENTRIES];
 static u64 T_byte_rook[4900], T_byte_bishop[1428];
 static int g_n_rook_unique, g_n_bishop_unique, g_fancy_used;
-static int g_backend = ORC_FANCY; /* reference default: switches.h:113-125 */
+/* slider backend of the calling thread; the reference default is fancy
+ * magics (switches.h:113-125).  Worker threads of orc_perft_batch inherit the
+ * caller's choice explicitly. */
+static __thread int g_backend __attribute__((tls_model("initial-exec"))) = ORC_FANCY;
 static int g_inited;
 
 /* MoveGeneratorBitboard.h:905-925: squaresInBetween (pure calculation) */
@@ -1138,6 +1141,7 @@ uint64_t orc_perft(const orc_pos *p, int depth) {
 typedef struct { const orc_pos *p; int n, depth, stride, start, backend; uint64_t *out; } job_t;
 static void *perft_worker(void *arg) {
     job_t *j = (job_t *)arg;
+    g_backend = j->backend;
     for (int i = j->start; i < j->n; i += j->stride) j->out[i] = orc_perft(&j->p[i], j->depth);
     return 0;
 }

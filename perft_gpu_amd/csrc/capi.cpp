@@ -196,7 +196,10 @@ pf_status pf_perft_batch_mode(pf_ctx* ctx, const pf_pos48* pos, int n, int depth
     if (!ctx || (n > 0 && (!pos || !out)) || n < 0 || check_depth(depth)) return PF_EINVAL;
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
     if (ctx->eng.serving) return PF_EBUSY;
-    if (mode == PF_MODE_LEVELS) return ctx->eng.perft_levels(pos, n, depth, out);
+    // the transposition table lives in the level driver; AUTO uses it when
+    // configured, PERSISTENT always runs the single table-free launch
+    if (mode == PF_MODE_LEVELS || (mode == PF_MODE_AUTO && ctx->eng.d_tt && depth >= 3))
+        return ctx->eng.perft_levels(pos, n, depth, out);
     pf_status s = ctx->eng.perft_persistent(pos, n, depth, out);
     if (s == PF_ENOMEM && mode == PF_MODE_AUTO) s = ctx->eng.perft_levels(pos, n, depth, out);
     return s;

@@ -88,6 +88,6 @@ def request_position(pos_seed, cfg=C4):
 # make+count at every node below the root; 64-bit and/or/xor/not/shift/add/
 # sub/select = 2, multiply = 4, popcount = 2, ctz = 3, table loads = 0),
 # summed over the whole C2 tree by the instrumented CPU restatement
-# (tests/golden/make_golden.py): 416,374,379,703 ops / 8,988,346,871 leaves.
-C2_OPS_PER_LEAF = 416374379703 / 8988346871
+# (tests/golden/make_golden.py): 416,777,397,783 ops / 8,988,346,871 leaves.
+C2_OPS_PER_LEAF = 416777397783 / 8988346871
 C2_LEAVES = 8988346871

@@ -57,6 +57,11 @@ def main():
     c = W.C3
     rows = o.playouts(c["seed"], c["n"], c["min_plies"], c["max_plies"], c["min_moves"], c["no_check"])
     g["c3"] = {"config": c["name"], "positions": [hexpos(r) for r in rows], "perft": counts(o, rows, range(1, 7))}
+    # depth 7 by root move: the children at depth 6, spread over the threads
+    import numpy as np
+    kids = o.children(o.from48(rows[0]))
+    karr = np.array([k.as_tuple() for k in kids], dtype=rows.dtype)
+    g["c3"]["perft"]["7"] = [int(o.perft_batch(karr, 6, nthreads=NTHREADS).sum())]
     print("c3 done %.1fs" % (time.time() - t0), flush=True)
 
     c = W.C5

@@ -226,3 +226,13 @@ def test_op_model(oracle):
     assert 10 < ops / leaves < 200
     ops5, leaves5 = oracle.opcount(p, 5)
     assert leaves5 == 4865609 and ops5 > ops
+
+
+def test_op_model_constant_matches_golden():
+    """the op-model constant used by bench.py is the one the fixture records"""
+    import json
+    from perft_gpu_amd import workloads as W
+    with open(os.path.join(GOLDEN, "workloads.json")) as f:
+        om = json.load(f)["c2"]["op_model"]
+    assert om["leaves"] == W.C2_LEAVES
+    assert abs(om["int32_lane_ops"] / om["leaves"] - W.C2_OPS_PER_LEAF) < 1e-12
