@@ -88,6 +88,7 @@ typedef struct pf_stats {
     double gpu_ms;            /* device time of expand kernels (HIP events) */
     double last_expand_ms;    /* average duration of the last run's expand launches */
     uint64_t last_expand_launches;
+    uint64_t dup_nodes;       /* nodes found to duplicate another node of their level (TT runs) */
 } pf_stats;
 
 /* Replaces initGPU + MoveGeneratorBitboard::init + setupHashTables128b
@@ -125,6 +126,10 @@ pf_status pf_apply_uci(pf_pos48 *pos, const char *uci);
 /* Slider backend selection at run time (the reference's switches.h:102-125
  * are compile-time). */
 pf_status pf_set_backend(pf_ctx *ctx, int backend);
+/* In-level duplicate elimination on runs with a transposition table (on by
+ * default): the compile-time FIND_DUPLICATES_IN_BFS of switches.h:50-52 as a
+ * run-time switch.  Counts are the same either way. */
+pf_status pf_set_dedup(pf_ctx *ctx, int on);
 pf_status pf_get_stats(pf_ctx *ctx, pf_stats *out);
 /* Wait for all device work of this context (the reference's cudaDeviceSynchronize
  * calls around its launches, launcher.h:1490-1492). */

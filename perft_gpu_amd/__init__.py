@@ -57,7 +57,7 @@ class Stats(ctypes.Structure):
     _fields_ = [("kernel_launches", ctypes.c_uint64), ("levels_run", ctypes.c_uint64), ("chunks", ctypes.c_uint64),
                 ("peak_arena_bytes", ctypes.c_uint64), ("tt_probes", ctypes.c_uint64), ("tt_hits", ctypes.c_uint64),
                 ("tt_stores", ctypes.c_uint64), ("gpu_ms", ctypes.c_double), ("last_expand_ms", ctypes.c_double),
-                ("last_expand_launches", ctypes.c_uint64)]
+                ("last_expand_launches", ctypes.c_uint64), ("dup_nodes", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -68,7 +68,7 @@ _lib = None
 # every symbol the C ABI declares (checked by the CPU test suite)
 EXPORTS = ["pf_init", "pf_destroy", "pf_strerror", "pf_fen_to_pos", "pf_pos_to_fen", "pf_perft",
            "pf_perft_batch", "pf_perft_batch_mode", "pf_divide", "pf_move_to_uci", "pf_apply_uci",
-           "pf_set_backend", "pf_get_stats", "pf_reset_stats", "pf_sync", "pf_probe", "pf_playouts",
+           "pf_set_backend", "pf_set_dedup", "pf_get_stats", "pf_reset_stats", "pf_sync", "pf_probe", "pf_playouts",
            "pf_count_moves_host", "pf_serve_start", "pf_submit", "pf_poll", "pf_serve_stop", "pf_now"]
 
 
@@ -105,30 +105,19 @@ def load_library(path=LIB_PATH):
         "pf_playouts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_void_p]),
         "pf_count_moves_host": (ctypes.c_int, [P(Pos48)]),
-    }
-    for name, (res, args) in sig.items():
-        fn = getattr(lib, name)
-        fn.restype = res
-        fn.argtypes = args
-    for name, (res, args) in _optional_sigs().items():
-        if hasattr(lib, name):
-            fn = getattr(lib, name)
-            fn.restype = res
-            fn.argtypes = args
-    _lib = lib
-    return lib
-
-
-def _optional_sigs():
-    P = ctypes.POINTER
-    return {
+        "pf_set_dedup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
         "pf_submit": (ctypes.c_int, [ctypes.c_void_p, P(Pos48), ctypes.c_int, ctypes.c_uint64]),
         "pf_poll": (ctypes.c_int, [ctypes.c_void_p, P(Result), ctypes.c_int]),
         "pf_serve_start": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64]),
         "pf_serve_stop": (ctypes.c_int, [ctypes.c_void_p]),
         "pf_now": (ctypes.c_double, []),
-        "pf_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
     }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
 
 
 def _strerror(status):
@@ -217,6 +206,9 @@ class Engine:
     def set_backend(self, backend):
         _check(self._lib.pf_set_backend(self._ctx, BACKENDS[backend] if isinstance(backend, str) else int(backend)),
                "set_backend")
+
+    def set_dedup(self, on):
+        _check(self._lib.pf_set_dedup(self._ctx, 1 if on else 0), "set_dedup")
 
     def perft(self, pos, depth):
         out = ctypes.c_uint64()

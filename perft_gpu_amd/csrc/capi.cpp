@@ -262,6 +262,13 @@ pf_status pf_set_backend(pf_ctx* ctx, int backend) {
     return PF_OK;
 }
 
+pf_status pf_set_dedup(pf_ctx* ctx, int on) {
+    if (!ctx) return PF_EINVAL;
+    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    ctx->eng.dedup = on != 0;
+    return PF_OK;
+}
+
 pf_status pf_get_stats(pf_ctx* ctx, pf_stats* out) {
     if (!ctx || !out) return PF_EINVAL;
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);

@@ -50,8 +50,9 @@ struct Engine {
     u64 tt_buckets = 0;
     ZTables h_z;
     ZTables* d_z = nullptr;
-    unsigned long long* d_ttstats = nullptr;  // probes, hits, stores
+    unsigned long long* d_ttstats = nullptr;  // probes, hits, stores, -, duplicates' moves, duplicates
     int tt_grid[4] = {};
+    bool dedup = true;  // in-level duplicate detection on TT runs
 
     struct TLevel {  // one breadth-first level with TT bookkeeping (80 B/node)
         Board* b;
@@ -81,7 +82,8 @@ struct Engine {
     pf_status descend(const Board* lvl, const u32* rts, u64 n, u64 n_next, int L);
     bool alloc_tlevel(u64 n, TLevel* lv);
     pf_status expand_tt(int mode, const TLevel& lv, u64 off, u64 m, const TLevel* nx, u64 cap, int L,
-                        u64* written, u64* next_total);
+                        u64* written, u64* next_total, DupEntry* dup = nullptr, u64 dup_n = 0);
+    pf_status dedup_level(const TLevel& lv, u64 n, DupEntry* dup, u64 dup_n, u64* removed_moves);
     pf_status descend_tt(const TLevel& lv, u64 off, u64 m, u64 n_next, int L);
   public:
     pf_status perft_levels_tt(const pf_pos48* pos, int n, int depth, u64* out);

@@ -36,8 +36,8 @@ pf_status Engine::init_tt(u64 bytes, u64 seed) {
     tt_buckets = buckets;
     PF_HIP(hipMalloc(&d_z, sizeof(ZTables)));
     PF_HIP(hipMemcpy(d_z, &h_z, sizeof(ZTables), hipMemcpyHostToDevice));
-    PF_HIP(hipMalloc(&d_ttstats, 4 * sizeof(unsigned long long)));
-    PF_HIP(hipMemset(d_ttstats, 0, 4 * sizeof(unsigned long long)));
+    PF_HIP(hipMalloc(&d_ttstats, 8 * sizeof(unsigned long long)));
+    PF_HIP(hipMemset(d_ttstats, 0, 8 * sizeof(unsigned long long)));
     for (int b = 0; b < 4; b++) tt_grid[b] = expand_tt_grid(b, n_cu);
     return tt_clear();
 }
@@ -60,7 +60,7 @@ bool Engine::alloc_tlevel(u64 n, TLevel* lv) {
 }
 
 pf_status Engine::expand_tt(int mode, const TLevel& lv, u64 off, u64 m, const TLevel* nx, u64 cap, int L,
-                            u64* written, u64* next_total) {
+                            u64* written, u64* next_total, DupEntry* dup, u64 dup_n) {
     if (!m) {
         if (written) *written = 0;
         if (next_total) *next_total = 0;
@@ -95,6 +95,8 @@ pf_status Engine::expand_tt(int mode, const TLevel& lv, u64 off, u64 m, const TL
     a.tile_ctr = d_ctrl + 1;
     a.err = d_ctrl + 0;
     a.stats = d_ttstats;
+    a.dup = dup;
+    a.dup_n = dup_n;
     if (timing) PF_HIP(hipEventRecord(ev0, stream));
     PF_HIP(launch_expand_tt(backend, d_tab, d_z, a, tt_grid[backend], stream));
     if (timing) PF_HIP(hipEventRecord(ev1, stream));
@@ -117,18 +119,55 @@ pf_status Engine::expand_tt(int mode, const TLevel& lv, u64 off, u64 m, const TL
     return PF_OK;
 }
 
+// Duplicates among the n nodes of a freshly written level whose children
+// claimed slots of `dup` (k_dedup); *removed_moves = the moves they no
+// longer contribute to the next level.
+pf_status Engine::dedup_level(const TLevel& lv, u64 n, DupEntry* dup, u64 dup_n, u64* removed_moves) {
+    *removed_moves = 0;
+    if (!n) return PF_OK;
+    PF_HIP(hipMemsetAsync(d_ttstats + 4, 0, 2 * sizeof(unsigned long long), stream));
+    PF_HIP(launch_dedup(lv.b, lv.k1, lv.k2, lv.acc, (u32)n, dup, dup_n, d_ttstats + 4, stream));
+    st.kernel_launches++;
+    PF_HIP(hipMemcpyAsync(h_pinned + 8, d_ttstats + 4, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    *removed_moves = h_pinned[8];
+    st.dup_nodes += h_pinned[9];
+    return PF_OK;
+}
+
+// Slots of the duplicate table for a level of n nodes at ply `ply` (0: none).
+// Linear probing at a load of at most 2/3; transpositions need 3 plies or more.
+static u64 dup_slots(bool on, u64 n, int ply) {
+    if (!on || ply < 3 || n < 2) return 0;
+    return std::min<u64>(n + n / 2 + 64, 0xFFFFFFFFull);
+}
+
 // Parents [off, off + m) of level `lv` (level L) have n_next children in all.
 pf_status Engine::descend_tt(const TLevel& lv, u64 off, u64 m, u64 n_next, int L) {
     if (!m) return PF_OK;
     if (depth - L == 2) return expand_tt(kLeaf, lv, off, m, nullptr, 0, L, nullptr, nullptr);
-    const u64 need = n_next * kTNodeBytes + 8 * kAlignTT;
+    const u64 dn = dup_slots(dedup, n_next, L + 1);
+    const u64 need = n_next * kTNodeBytes + dn * sizeof(DupEntry) + 9 * kAlignTT;
     const u64 free_b = arena_bytes - arena_top;
     if (need <= free_b && n_next <= 0xFFFFFFFFull) {
         char* mark = arena + arena_top;
         TLevel nx;
         if (!alloc_tlevel(n_next, &nx)) { release(mark); return PF_ENOMEM; }
+        // the duplicate table lives above the new level until k_dedup has run
+        char* dmark = arena + arena_top;
+        DupEntry* dup = dn ? reinterpret_cast<DupEntry*>(alloc(dn * sizeof(DupEntry))) : nullptr;
+        if (dn && !dup) { release(mark); return PF_ENOMEM; }
+        pf_status s = PF_OK;
+        if (dup && hipMemsetAsync(dup, 0, dn * sizeof(DupEntry), stream) != hipSuccess) s = PF_EDEVICE;
         u64 written = 0, n_next2 = 0;
-        pf_status s = expand_tt(kWrite, lv, off, m, &nx, n_next, L, &written, &n_next2);
+        if (s == PF_OK) s = expand_tt(kWrite, lv, off, m, &nx, n_next, L, &written, &n_next2, dup, dn);
+        if (s == PF_OK && dup) {
+            u64 removed = 0;
+            s = dedup_level(nx, written, dup, dn, &removed);
+            n_next2 -= removed;
+        }
+        release(dmark);
         if (s == PF_OK) s = descend_tt(nx, 0, written, n_next2, L + 1);
         if (s == PF_OK && written) {
             if (launch_upsweep(nx.parent, nx.k1, nx.k2, nx.acc, (u32)written, lv.acc, d_tt, tt_buckets - 1,

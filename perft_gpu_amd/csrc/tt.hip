@@ -63,6 +63,28 @@ __device__ __forceinline__ void tt_store(uint4* tt, u64 mask, u64 k1, u64 k2, in
     else bkt[1] = ne;              // always-replace slot
 }
 
+// Slot of a key in a duplicate table of n slots (n < 2^32): multiply-shift
+// range reduction of the low half of the first key, so that the high half
+// stays free to verify a match.
+__device__ __forceinline__ u64 dup_slot(u64 k1, u64 n) { return ((k1 & 0xFFFFFFFFull) * n) >> 32; }
+
+// A written child claims the first empty slot of its probe sequence, unless a
+// node with its second key already holds one.  The claimant publishes its
+// index; other nodes read it only in a later launch (k_dedup).
+__device__ __forceinline__ void dup_insert(DupEntry* t, u64 n, u64 k1, u64 k2, u32 idx) {
+    if (!k2) return;
+    u64 s = dup_slot(k1, n);
+    for (int p = 0; p < kDupProbes; p++) {
+        const unsigned long long old = atomicCAS(&t[s].key, 0ull, (unsigned long long)k2);
+        if (old == 0) {
+            t[s].val = (k1 & 0xFFFFFFFF00000000ull) | idx;
+            return;
+        }
+        if (old == k2) return;
+        s = (s + 1 == n) ? 0 : s + 1;
+    }
+}
+
 struct TTLds {
     Board b[kTTP];
     u32 root[kTTP];
@@ -193,6 +215,7 @@ __device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTa
                             }
                             a.out_parent[pos] = a.in_base + p0 + par;
                             a.out_acc[pos] = 0;
+                            if (a.dup) dup_insert(a.dup, a.dup_n, ck1, ck2, (u32)pos);
                         } else {
                             atomicOr(a.err, 1u);
                         }
@@ -269,16 +292,61 @@ __global__ __launch_bounds__(256) void k_roots_tt(const SliderTables* tabs, Boar
     atomicAdd(total, (unsigned long long)c);
 }
 
-// Finished level: add each node's count to its parent and store it.
+// Duplicates within a freshly written level (replaces writeIndexInHashForDuplicates
+// + checkandMarkDuplicates, perft_bb.h:788-846): a node whose keys match the
+// claimant of its slot, and is not the claimant, is a duplicate.  It loses its
+// moves (count byte 0: nothing below it is expanded) and its accumulator
+// records the original, whose exact count it takes in the upsweep
+// (perft_bb.h:871-878).  removed[0] += the duplicates' move counts,
+// removed[1] += the number of duplicates.
+__global__ __launch_bounds__(256) void k_dedup(Board* b, const u64* k1, const u64* k2, unsigned long long* acc, u32 n,
+                                               const DupEntry* t, u64 tn, unsigned long long* removed) {
+    const u32 i = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long mv = 0, nd = 0;
+    if (i < n) {
+        const u64 key2 = k2[i];
+        if (key2) {
+            const u64 key1 = k1[i];
+            u64 s = dup_slot(key1, tn);
+            for (int p = 0; p < kDupProbes; p++) {
+                const DupEntry e = t[s];
+                if (e.key == 0) break;
+                if (e.key == key2) {
+                    const u32 o = (u32)e.val;
+                    if ((e.val >> 32) == (key1 >> 32) && o != i && o < n) {
+                        const u64 pw = b[i].pawns;
+                        mv = pw >> 56;
+                        b[i].pawns = pw & ~(0xFFull << 56);
+                        acc[i] = kDupBit | o;
+                        nd = 1;
+                    }
+                    break;
+                }
+                s = (s + 1 == tn) ? 0 : s + 1;
+            }
+        }
+    }
+    mv = t_wave_sum64(mv);
+    nd = t_wave_sum64(nd);
+    if ((threadIdx.x & 63) == 0 && nd) {
+        atomicAdd(&removed[0], mv);
+        atomicAdd(&removed[1], nd);
+    }
+}
+
+// Finished level: add each node's count to its parent and store it.  A
+// duplicate takes its original's count and stores nothing.
 __global__ __launch_bounds__(256) void k_upsweep(const u32* parent, const u64* k1, const u64* k2,
                                                  const unsigned long long* acc, u32 n, unsigned long long* parent_acc,
                                                  uint4* tt, u64 mask, int depth, unsigned long long* stores) {
     const u32 i = blockIdx.x * 256 + threadIdx.x;
     unsigned long long st = 0;
     if (i < n) {
-        const unsigned long long v = acc[i];
+        unsigned long long v = acc[i];
+        const bool dup = (v & kDupBit) != 0;
+        if (dup) v = acc[(u32)v];
         if (parent_acc) atomicAdd(&parent_acc[parent[i]], v);
-        if (tt) {
+        if (tt && !dup) {
             tt_store(tt, mask, k1[i], k2[i], depth, v);
             st = 1;
         }
@@ -434,6 +502,13 @@ hipError_t launch_roots_tt(int backend, const SliderTables* tabs, Board* roots, 
     case kPlain: hipLaunchKernelGGL(k_roots_tt<kPlain>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total); break;
     default: hipLaunchKernelGGL(k_roots_tt<kFancy>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total); break;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_dedup(Board* b, const u64* k1, const u64* k2, unsigned long long* acc, u32 n, const DupEntry* t,
+                        u64 tn, unsigned long long* removed, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_dedup, dim3((n + 255) / 256), dim3(256), 0, s, b, k1, k2, acc, n, t, tn, removed);
     return hipGetLastError();
 }
 

@@ -11,6 +11,16 @@ namespace pf {
 template <int B>
 using SlOfT = SlOf<B>;
 
+// In-level duplicate detection (open addressing, linear probing): `key` is
+// the node's second Zobrist key (0 = empty slot, claimed by compare-and-swap),
+// `val` = high half of the first key << 32 | the claimant's index in its level.
+struct alignas(16) DupEntry {
+    unsigned long long key;
+    unsigned long long val;
+};
+constexpr int kDupProbes = 32;
+constexpr unsigned long long kDupBit = 1ULL << 63;  // acc of a duplicate: kDupBit | original's index
+
 struct TTArgs {
     // parents: a contiguous range of one level
     const Board* in;
@@ -37,6 +47,8 @@ struct TTArgs {
     u32* tile_ctr;
     u32* err;
     unsigned long long* stats;   // [0] probes, [1] hits
+    DupEntry* dup;               // written children claim a slot here (null: no duplicate detection)
+    u64 dup_n;                   // slots (< 2^32)
 };
 
 // One serving step (serve.cpp): expand/leaf segments of many jobs, their
@@ -95,6 +107,8 @@ hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables
 hipError_t launch_roots_tt(int backend, const SliderTables* tabs, Board* roots, const u64* k1, const u64* k2, u32 n,
                            int depth, const uint4* tt, u64 mask, unsigned long long* acc, unsigned long long* total,
                            hipStream_t s);
+hipError_t launch_dedup(Board* b, const u64* k1, const u64* k2, unsigned long long* acc, u32 n, const DupEntry* t,
+                        u64 tn, unsigned long long* removed, hipStream_t s);
 hipError_t launch_upsweep(const u32* parent, const u64* k1, const u64* k2, const unsigned long long* acc, u32 n,
                           unsigned long long* parent_acc, uint4* tt, u64 mask, int depth, unsigned long long* stores,
                           hipStream_t s);

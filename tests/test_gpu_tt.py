@@ -91,3 +91,45 @@ def test_tt_tiny_table_heavy_replacement(pg, oracle):
         assert np.array_equal(eng.perft_batch(pos, 5), oracle.perft_batch(pos, 5, nthreads=8))
     finally:
         eng.close()
+
+
+def test_dedup_on_off_equal_known_answer(pg):
+    """in-level duplicate elimination: same count on and off, and it finds
+    duplicates (start position: 8,902 nodes at ply 3, far fewer distinct)"""
+    start = pg.fen_to_pos(pg.START_FEN)
+    for on in (True, False):
+        eng = pg.Engine(backend="ks", arena_bytes=2 << 30, tt_bytes=256 << 20, zobrist_seed=11)
+        try:
+            eng.set_dedup(on)
+            assert eng.perft(start, 6) == 119060324
+            dups = eng.stats()["dup_nodes"]
+            assert (dups > 0) if on else (dups == 0), (on, dups)
+        finally:
+            eng.close()
+
+
+def test_dedup_across_roots_of_a_batch(pg, oracle):
+    """equal roots in one batch: every node below the first copy's ply 3 is a
+    duplicate of a node of the other copy, and both roots get the full count"""
+    pos = pg.playouts(4711, 6, 10, 30)
+    batch = np.concatenate([pos, pos, pos[::-1]])
+    eng = pg.Engine(backend="fancy", arena_bytes=4 << 30, tt_bytes=128 << 20, zobrist_seed=5)
+    try:
+        got = eng.perft_batch(batch, 5)
+        want = oracle.perft_batch(pos, 5, nthreads=8)
+        assert np.array_equal(got[:6], want) and np.array_equal(got[6:12], want)
+        assert np.array_equal(got[12:], want[::-1])
+        assert eng.stats()["dup_nodes"] > 0
+    finally:
+        eng.close()
+
+
+def test_dedup_c3_golden_fresh_table(pg):
+    g = _golden()["c3"]
+    pos = _rows(pg, g["positions"])
+    eng = pg.Engine(backend="ks", arena_bytes=16 << 30, tt_bytes=1 << 30, zobrist_seed=0xC3C3)
+    try:
+        assert int(eng.perft_batch(pos, 6)[0]) == g["perft"]["6"][0]
+        assert eng.stats()["dup_nodes"] > 0
+    finally:
+        eng.close()
