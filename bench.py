@@ -16,9 +16,12 @@ Also reported on the same box, outside the timed region:
   serving       C4: Poisson arrivals (50 req/s) replayed in real time for
                 --serve-seconds, depths {3:.30,4:.30,5:.25,6:.12,7:.03},
                 32 GiB shared TT: p50/p99 latency and nodes/sec
-  c3            one 20-30-ply middlegame position, perft 8, 64 GiB TT
-  c5            the first --c5-positions of the 4096 high-mobility positions,
-                perft 6, 128 GiB TT
+  c3            one 20-30-ply middlegame position, perft 8, 64 GiB TT with
+                in-level duplicate elimination (and perft 7 against the
+                fixture, and perft 8 with the TT off); HBM random-access
+                roofline from the live random-access and copy probes
+  c5            the 4096 high-mobility positions (--c5-positions), perft 6,
+                128 GiB TT
   cpu_baseline  the oracle (CPU restatement of the reference perft, fancy
                 magics) on the host cores, same C2 positions, same depth
 """
@@ -81,7 +84,7 @@ def run_serving(pg, W, seconds, backend, paced=True):
     reqs = [r for r in reqs if r[0] < seconds]
     poss = [W.request_position(s) for _, _, s in reqs]
     eng = pg.Engine(backend=backend, arena_bytes=96 << 30, tt_bytes=cfg["tt_bytes"], zobrist_seed=cfg["seed"])
-    eng.serve_start(max_jobs=128)
+    eng.serve_start(max_jobs=256)
     results = {}
     done = threading.Event()
 
@@ -104,6 +107,7 @@ def run_serving(pg, W, seconds, backend, paced=True):
         eng.submit(p, d, i)
     done.set()
     th.join(timeout=600)
+    st = eng.stats()
     eng.serve_stop()
     eng.close()
     lat = sorted(r["done_s"] - r["submit_s"] for r in results.values())
@@ -120,9 +124,12 @@ def run_serving(pg, W, seconds, backend, paced=True):
     return {"paced": paced, "requests": len(reqs), "completed": len(results), "errors": bad,
             "p50_ms": pct(lat, 0.50), "p99_ms": pct(lat, 0.99), "max_ms": lat[-1] * 1e3 if lat else None,
             "nodes": nodes, "nodes_per_s": nodes / span if span > 0 else None, "span_s": span,
-            "arrival_rate_per_s": cfg["rate"], "seconds": seconds,
+            "arrival_rate_per_s": cfg["rate"], "seconds": seconds, "steps": st["levels_run"],
+            "step_busy_s": st["gpu_ms"] * 1e-3,
+            "dup_nodes": st["dup_nodes"],
             "p50_ms_by_depth": {str(d): pct(sorted(v), 0.5) for d, v in sorted(by_depth.items())},
-            "p99_ms_by_depth": {str(d): pct(sorted(v), 0.99) for d, v in sorted(by_depth.items())}}
+            "p99_ms_by_depth": {str(d): pct(sorted(v), 0.99) for d, v in sorted(by_depth.items())},
+            "_counts": [results[i]["nodes"] if i in results else None for i in range(len(reqs))]}
 
 
 def run_plain_requests(pg, W, seconds, backend):
@@ -132,27 +139,62 @@ def run_plain_requests(pg, W, seconds, backend):
     reqs = [r for r in W.serving_requests(cfg) if r[0] < seconds]
     eng = pg.Engine(backend=backend, arena_bytes=96 << 30, tt_bytes=cfg["tt_bytes"], zobrist_seed=cfg["seed"])
     try:
-        nodes = 0
+        counts = []
+        by_depth = {}
         t = time.perf_counter()
         for _, d, s in reqs:
-            nodes += eng.perft(pg.Pos48(*[int(x) for x in W.request_position(s)]), d)
+            t1 = time.perf_counter()
+            counts.append(eng.perft(pg.Pos48(*[int(x) for x in W.request_position(s)]), d))
+            by_depth[str(d)] = by_depth.get(str(d), 0.0) + time.perf_counter() - t1
         el = time.perf_counter() - t
-        return {"requests": len(reqs), "nodes": nodes, "seconds": el, "nodes_per_s": nodes / el}
+        nodes = sum(counts)
+        return {"requests": len(reqs), "nodes": nodes, "seconds": el, "nodes_per_s": nodes / el,
+                "seconds_by_depth": by_depth, "_counts": counts}
     finally:
         eng.close()
 
 
-def run_tt_workload(pg, W, cfg, positions, depth, backend, arena, check=None):
-    eng = pg.Engine(backend=backend, arena_bytes=arena, tt_bytes=cfg["tt_bytes"], zobrist_seed=cfg["seed"])
+def measure_ceilings(pg):
+    """Random 16-byte read-modify-writes per second over a 16 GiB table, and
+    streaming copy bytes per second (read + write), on this device."""
+    eng = pg.Engine(arena_bytes=64 << 20)
+    try:
+        return {"random16_per_s": eng.probe(2, 16 << 30, 0), "copy_bytes_per_s": eng.probe(1, 1 << 30, 0)}
+    finally:
+        eng.close()
+
+
+FRONTIER_NODE_BYTES = 80  # board 48 + root word 4 + two keys 16 + parent 4 + accumulator 8
+
+
+def tt_roofline(st, seconds, ceil):
+    """HBM random-access roofline of a TT run: every probe, store, duplicate-
+    slot claim and duplicate-slot lookup is one random access of at most
+    32 bytes, priced at the measured random 16-byte rate; every frontier node
+    is written once and read once at streaming rate.  t_min / t_run."""
+    rnd = st["tt_probes"] + st["tt_stores"] + 2 * st["frontier_nodes"]
+    stream = 2 * FRONTIER_NODE_BYTES * st["frontier_nodes"]
+    t_min = rnd / ceil["random16_per_s"] + stream / ceil["copy_bytes_per_s"]
+    return {"bound": "hbm-random", "random_accesses": rnd, "stream_bytes": stream, "t_min_s": t_min,
+            "random16_per_s": ceil["random16_per_s"], "copy_bytes_per_s": ceil["copy_bytes_per_s"],
+            "frac": t_min / seconds}
+
+
+def run_tt_workload(pg, W, cfg, positions, depth, backend, arena, check=None, ceil=None, tt=True):
+    eng = pg.Engine(backend=backend, arena_bytes=arena, tt_bytes=cfg["tt_bytes"] if tt else 0,
+                    zobrist_seed=cfg["seed"])
     try:
         t = time.perf_counter()
         got = eng.perft_batch(positions, depth)
         el = time.perf_counter() - t
         st = eng.stats()
         out = {"depth": depth, "positions": len(positions), "nodes": int(got.sum()), "seconds": el,
-               "nodes_per_s": float(got.sum()) / el, "tt_gib": cfg["tt_bytes"] >> 30,
+               "nodes_per_s": float(got.sum()) / el, "tt_gib": (cfg["tt_bytes"] >> 30) if tt else 0,
                "tt_probes": st["tt_probes"], "tt_hits": st["tt_hits"], "tt_stores": st["tt_stores"],
+               "dup_nodes": st["dup_nodes"], "frontier_nodes": st["frontier_nodes"],
                "chunks": st["chunks"], "peak_arena_gib": st["peak_arena_bytes"] / 2 ** 30}
+        if tt and ceil:
+            out["roofline"] = tt_roofline(st, el, ceil)
         if check is not None:
             out["counts_match"] = bool(check(got))
         return out
@@ -167,7 +209,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--backend", default="ks")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
-    ap.add_argument("--c5-positions", type=int, default=256)
+    ap.add_argument("--c5-positions", type=int, default=4096)
     ap.add_argument("--no-extras", action="store_true", help="skip every comparison run")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -258,10 +300,13 @@ def main():
     if not args.no_extras:
         try:
             line["serving"] = run_serving(pg, W, args.serve_seconds, args.backend)
-            log("serving: %s" % line["serving"])
+            log("serving: p50 %.3f ms p99 %.3f ms" % (line["serving"]["p50_ms"], line["serving"]["p99_ms"]))
             cap = run_serving(pg, W, args.serve_seconds, args.backend, paced=False)
             line["serving_capacity"] = cap
             pl = run_plain_requests(pg, W, args.serve_seconds, args.backend)
+            sc, cc, pc = line["serving"].pop("_counts"), cap.pop("_counts"), pl.pop("_counts")
+            line["serving"]["counts_match_plain"] = sc == pc
+            cap["counts_match_plain"] = cc == pc
             line["plain_same_requests"] = pl
             line["served_over_plain"] = cap["nodes_per_s"] / pl["nodes_per_s"]
             log("serving capacity %.3e nodes/s vs plain %.3e" % (cap["nodes_per_s"], pl["nodes_per_s"]))
@@ -271,15 +316,22 @@ def main():
             import json as _j
             with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
                 g = _j.load(f)
+            ceil = measure_ceilings(pg)
+            log("ceilings: %s" % ceil)
             c3 = W.positions(W.C3)
-            line["c3"] = run_tt_workload(pg, W, W.C3, c3, 7, args.backend, 128 << 30,
-                                         check=lambda got: int(got[0]) == g["c3"]["perft"]["7"][0])
-            line["c3"]["d8"] = run_tt_workload(pg, W, W.C3, c3, 8, args.backend, 128 << 30)
+            line["c3"] = run_tt_workload(pg, W, W.C3, c3, 8, args.backend, 128 << 30, ceil=ceil)
+            line["c3"]["d7"] = run_tt_workload(pg, W, W.C3, c3, 7, args.backend, 128 << 30, ceil=ceil,
+                                               check=lambda got: int(got[0]) == g["c3"]["perft"]["7"][0])
+            line["c3"]["d8_tt_off"] = run_tt_workload(pg, W, W.C3, c3, 8, args.backend, 128 << 30, tt=False)
+            line["c3"]["speedup_over_tt_off"] = line["c3"]["d8_tt_off"]["seconds"] / line["c3"]["seconds"]
+            line["c3"]["counts_match_tt_off"] = line["c3"]["nodes"] == line["c3"]["d8_tt_off"]["nodes"]
             log("c3: %s" % line["c3"])
-            c5 = W.positions(W.C5)[:args.c5_positions]
-            line["c5"] = run_tt_workload(pg, W, W.C5, c5, 6, args.backend, 96 << 30,
-                                         check=lambda got: [int(x) for x in got[:8]] == g["c5"]["perft_first8_d6"])
-            log("c5: %s" % line["c5"])
+            if args.c5_positions > 0:
+                c5 = W.positions(W.C5)[:args.c5_positions]
+                line["c5"] = run_tt_workload(
+                    pg, W, W.C5, c5, 6, args.backend, 96 << 30, ceil=ceil,
+                    check=lambda got: [int(x) for x in got[:8]] == g["c5"]["perft_first8_d6"][:len(got)])
+                log("c5: %s" % line["c5"])
         except Exception as e:
             line["tt_runs_error"] = repr(e)
     if not args.no_cpu:

@@ -89,6 +89,7 @@ typedef struct pf_stats {
     double last_expand_ms;    /* average duration of the last run's expand launches */
     uint64_t last_expand_launches;
     uint64_t dup_nodes;       /* nodes found to duplicate another node of their level (TT runs) */
+    uint64_t frontier_nodes;  /* nodes written to frontier levels (TT runs) */
 } pf_stats;
 
 /* Replaces initGPU + MoveGeneratorBitboard::init + setupHashTables128b

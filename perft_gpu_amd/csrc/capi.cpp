@@ -273,6 +273,14 @@ pf_status pf_get_stats(pf_ctx* ctx, pf_stats* out) {
     if (!ctx || !out) return PF_EINVAL;
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
     *out = ctx->eng.st;
+    if (ctx->srv) {  // the serving loop keeps its own counters
+        u64 steps = 0, segs = 0, dups = 0;
+        double busy = 0;
+        ctx->srv->counters(&steps, &segs, &busy, &dups);
+        out->levels_run += steps;
+        out->dup_nodes += dups;
+        out->gpu_ms += busy * 1e3;  // step graphs, launch to completion
+    }
     return PF_OK;
 }
 

@@ -108,6 +108,7 @@ pf_status Engine::expand_tt(int mode, const TLevel& lv, u64 off, u64 m, const TL
     const u32* c = reinterpret_cast<const u32*>(h_pinned);
     if (c[0]) return PF_ENOMEM;
     if (written) *written = c[2];
+    if (mode == kWrite) st.frontier_nodes += c[2];
     if (next_total) *next_total = h_pinned[4];
     if (timing) {
         float ms = 0;
@@ -137,10 +138,9 @@ pf_status Engine::dedup_level(const TLevel& lv, u64 n, DupEntry* dup, u64 dup_n,
 }
 
 // Slots of the duplicate table for a level of n nodes at ply `ply` (0: none).
-// Linear probing at a load of at most 2/3; transpositions need 3 plies or more.
 static u64 dup_slots(bool on, u64 n, int ply) {
-    if (!on || ply < 3 || n < 2) return 0;
-    return std::min<u64>(n + n / 2 + 64, 0xFFFFFFFFull);
+    if (!on || ply < kDupMinPly || n < 2) return 0;
+    return dup_table_slots(n);
 }
 
 // Parents [off, off + m) of level `lv` (level L) have n_next children in all.

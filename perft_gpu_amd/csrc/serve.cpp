@@ -1,24 +1,35 @@
 // serve.cpp -- continuous-batching perft server (no reference counterpart: the
 // reference answers one perft per process run, perft.cu:133-137).
 //
-// Requests (position, depth) are admitted while a job slot is free; each slot
-// owns an equal share of the frontier arena, so admission is by frontier
-// memory and a job never waits on another job's memory.  A job is the
-// depth-first chunked level driver of engine_tt.cpp rewritten as an explicit
-// frame stack, so that it can advance one GPU action at a time.
+// Admission is by predicted frontier memory: a request asks for the bytes its
+// breadth-first levels are predicted to need (root move count as branching
+// factor, the levels down to depth-2 plus a duplicate table, clamped to
+// [1 MiB, arena/4]) and is admitted when a job slot is free and a region of
+// that size is free in the arena (first fit; a younger request may pass an
+// older one that does not fit, unless the older one has waited 50 ms).  A
+// job never waits on another job's memory once admitted: where the
+// prediction falls short it chunks depth-first inside its own region.  With
+// a fixed job_bytes every job gets a region of exactly that size.  A job is
+// the chunked level driver of engine_tt.cpp rewritten as an explicit frame
+// stack, so that it can advance one GPU action at a time.
 //
 // One scheduler step = one replay of a captured graph:
 //   H2D step table -> roots kernel (new requests: table probe at full depth,
-//   move count) -> step kernel (the expand/leaf levels of every active job in
-//   one persistent launch over a shared tile queue) -> upsweep kernel (the
-//   finished levels of every job) -> D2H of per-segment outcomes and of the
-//   slot accumulators.
+//   move count) -> fill kernel (empties the duplicate tables of this step's
+//   write segments) -> step kernel (the expand/leaf levels of every active
+//   job in one persistent launch over a shared tile queue) -> duplicate
+//   kernel (in-level duplicates of every written level, as in the level
+//   driver) -> upsweep kernel (the finished levels of every job) -> D2H of
+//   per-segment outcomes and of the slot accumulators.
 // Requests that arrive while a step runs join the next step; their work
 // overlaps on the device with everyone else's.  All jobs share the lockless
 // transposition table.
 #include "serve.h"
 
 #include <string.h>
+
+#include "movegen.h"
+#include "sliders.h"
 
 #include <algorithm>
 #include <chrono>
@@ -42,9 +53,12 @@ Server::~Server() { stop(); }
 pf_status Server::start() {
     if (running_) return PF_OK;
     if (max_jobs_ < 1 || max_jobs_ > kMaxRoots) return PF_EINVAL;
-    if (!job_bytes_) job_bytes_ = e_.arena_bytes / (u64)max_jobs_;
     job_bytes_ &= ~(kAl - 1);
-    if (job_bytes_ * (u64)max_jobs_ > e_.arena_bytes || job_bytes_ < (1u << 20)) return PF_ENOMEM;
+    if (job_bytes_ && (job_bytes_ * (u64)max_jobs_ > e_.arena_bytes || job_bytes_ < (1u << 20))) return PF_ENOMEM;
+    if (e_.arena_bytes < (4u << 20)) return PF_ENOMEM;
+    free_mem_.clear();
+    free_mem_[0] = e_.arena_bytes & ~(kAl - 1);
+    max_region_ = job_bytes_ ? job_bytes_ : ((e_.arena_bytes / 4) & ~(kAl - 1));
     if (hipSetDevice(e_.device) != hipSuccess) return PF_EDEVICE;
     if (hipHostMalloc(&h_tab_, sizeof(StepTable), hipHostMallocDefault) != hipSuccess) return PF_ENOMEM;
     if (hipHostMalloc(&h_acc_, sizeof(u64) * max_jobs_, hipHostMallocDefault) != hipSuccess) return PF_ENOMEM;
@@ -110,6 +124,55 @@ void Server::stop() {
     root_p_ = nullptr; slot_acc_ = nullptr; h_tab_ = nullptr; h_acc_ = nullptr;
 }
 
+// Predicted peak frontier bytes of a request: the written levels are plies
+// 1..depth-2 (the last is expanded in leaf mode), each with 80 B per node plus
+// a 24-B duplicate-table share, grown by the root's move count per ply (held
+// in [16, 48]) and a quarter of headroom.
+u64 Server::predict_bytes(const pf_pos48& pos, int depth) const {
+    if (job_bytes_) return job_bytes_;
+    const u64 c = count_moves(to_board(pos), KSBackend{});
+    const double bf = (double)std::min<u64>(48, std::max<u64>(16, c));
+    // (duplicate elimination makes the deep levels smaller, but a region that
+    // proves too small costs many chunked steps, so no credit is taken for it)
+    double lvl = (double)c, sum = 0;
+    for (int k = 1; k <= depth - 2; k++) {
+        sum += lvl;
+        lvl *= bf;
+    }
+    double bytes = sum * (double)(kTNode + 24) * 1.25 + 16.0 * kAl;
+    const double lo = (double)(1u << 20), hi = (double)max_region_;
+    bytes = std::min(hi, std::max(lo, bytes));
+    return ((u64)bytes + kAl - 1) & ~(kAl - 1);
+}
+
+bool Server::region_alloc(u64 bytes, u64* off) {
+    for (auto it = free_mem_.begin(); it != free_mem_.end(); ++it) {
+        if (it->second < bytes) continue;
+        *off = it->first;
+        const u64 rest = it->second - bytes, at = it->first + bytes;
+        free_mem_.erase(it);
+        if (rest) free_mem_[at] = rest;
+        return true;
+    }
+    return false;
+}
+
+void Server::region_free(u64 off, u64 bytes) {
+    auto it = free_mem_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_mem_.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        free_mem_.erase(nx);
+    }
+    if (it != free_mem_.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            free_mem_.erase(it);
+        }
+    }
+}
+
 pf_status Server::submit(const pf_pos48& pos, int depth, u64 tag) {
     if (depth < 0 || depth > 16) return PF_EINVAL;
     Job j;
@@ -117,6 +180,7 @@ pf_status Server::submit(const pf_pos48& pos, int depth, u64 tag) {
     j.depth = depth;
     j.pos = pos;
     j.t_submit = now_s();
+    j.want = predict_bytes(pos, depth);
     {
         std::lock_guard<std::mutex> g(mu_);
         if (!running_ || stop_) return PF_EBUSY;
@@ -135,11 +199,12 @@ int Server::poll(pf_result* out, int max) {
     return n;
 }
 
-void Server::counters(u64* steps, u64* segs, double* busy) {
+void Server::counters(u64* steps, u64* segs, double* busy, u64* dups) {
     std::lock_guard<std::mutex> g(mu_);
     *steps = steps_;
     *segs = segs_;
     *busy = busy_;
+    *dups = dups_;
 }
 
 void Server::finish(Job& j, u64 nodes, int status) {
@@ -157,29 +222,78 @@ void Server::finish(Job& j, u64 nodes, int status) {
         done_.push_back(r);
         if (j.slot >= 0) free_slots_.push_back(j.slot);
     }
+    if (j.cap) region_free(j.base, j.cap);
+    j.cap = 0;
     j.slot = -1;
 }
 
+char* Server::take(Job& j, u64 bytes) {
+    bytes = (bytes + kAl - 1) & ~(kAl - 1);
+    if (j.top + bytes > j.cap) return nullptr;
+    char* p = e_.arena + j.base + j.top;
+    j.top += bytes;
+    return p;
+}
+
 bool Server::alloc_level(Job& j, u64 n, Engine::TLevel* lv) {
-    char* base = e_.arena + (u64)j.slot * job_bytes_;
-    auto take = [&](u64 bytes) -> char* {
-        bytes = (bytes + kAl - 1) & ~(kAl - 1);
-        if (j.top + bytes > job_bytes_) return nullptr;
-        char* p = base + j.top;
-        j.top += bytes;
-        return p;
-    };
-    lv->b = reinterpret_cast<Board*>(take(n * sizeof(Board)));
-    lv->root = reinterpret_cast<u32*>(take(n * 4));
-    lv->k1 = reinterpret_cast<u64*>(take(n * 8));
-    lv->k2 = reinterpret_cast<u64*>(take(n * 8));
-    lv->parent = reinterpret_cast<u32*>(take(n * 4));
-    lv->acc = reinterpret_cast<unsigned long long*>(take(n * 8));
+    lv->b = reinterpret_cast<Board*>(take(j, n * sizeof(Board)));
+    lv->root = reinterpret_cast<u32*>(take(j, n * 4));
+    lv->k1 = reinterpret_cast<u64*>(take(j, n * 8));
+    lv->k2 = reinterpret_cast<u64*>(take(j, n * 8));
+    lv->parent = reinterpret_cast<u32*>(take(j, n * 4));
+    lv->acc = reinterpret_cast<unsigned long long*>(take(j, n * 8));
     return lv->b && lv->root && lv->k1 && lv->k2 && lv->parent && lv->acc;
 }
 
-// Advance job j to its next GPU action.  Returns true if it added a segment
-// to the step table; finishes the job when its frame stack empties.
+// Frames st[k], st[k-1], ... have complete child levels: schedule their
+// upsweeps in rounds 0, 1, ... of this step, up to a frame that still has
+// chunks to run (its next chunk reuses the memory being upswept, so it waits
+// for the next step).  Returns the number scheduled.
+int Server::chain_up(Job& j, int k) {
+    int n = 0;
+    for (; k >= 0 && n < kUpRounds; k--) {
+        if (nup_planned_ >= (u32)kMaxUps) break;
+        Frame& p = j.st[(size_t)k];
+        UpSeg u;
+        u.parent = p.nx.parent;
+        u.k1 = p.nx.k1;
+        u.k2 = p.nx.k2;
+        u.acc = p.nx.acc;
+        u.parent_acc = p.lv.acc;
+        u.n = (u32)p.written;
+        u.depth = j.depth - p.L - 1;
+        up_[(size_t)n].push_back(u);
+        nup_planned_++;
+        n++;
+        if (p.chunking && p.off + p.m < p.end) break;
+    }
+    return n;
+}
+
+// After a step that upswept the child levels of the top n frames (chain_up):
+// free those levels, continue a chunked frame or pop finished ones.  Returns
+// true when the job's stack emptied (its root count is final).
+bool Server::after_chain(Job& j, int n) {
+    for (int i = 0; i < n; i++) {
+        Frame& p = j.st.back();
+        if (p.nx.b) j.top = p.mark;
+        p.nx = Engine::TLevel{};
+        if (p.chunking && p.off + p.m < p.end) {
+            p.off += p.m;
+            p.state = ST_START;
+            return false;
+        }
+        j.st.pop_back();
+        if (j.st.empty()) return true;
+    }
+    if (j.st.empty()) return true;
+    j.st.back().state = ST_CHILD;  // its child level is complete, not yet upswept
+    return false;
+}
+
+// Advance job j to its next GPU action (an expand/leaf segment, possibly with
+// a chain of upsweeps, or a chain of upsweeps alone).  Returns true if it
+// planned one; finishes the job when its frame stack empties.
 bool Server::plan(Job& j) {
     StepTable& t = *h_tab_;
     for (;;) {
@@ -212,21 +326,29 @@ bool Server::plan(Job& j) {
             a.out_count = &d_tab_->out[si].out_count;
             a.err = &d_tab_->out[si].err;
             a.tile_ctr = nullptr;
+            j.chain = 0;
             if (r == 2) {
+                // leaf pass over this level; the levels above it that it
+                // completes are upswept in the same step
                 a.mode = kLeaf;
                 f.state = ST_LEAF;
                 f.nx = Engine::TLevel{};
+                j.chain = chain_up(j, (int)j.st.size() - 2);
             } else {
-                const u64 free_b = job_bytes_ - j.top;
+                // children at ply L+1 get a duplicate table above their level
+                // for the length of this step
+                const bool dd = e_.dedup && e_.d_tt && f.L + 1 >= kDupMinPly;  // keys exist only with a table
+                const u64 node_b = kTNode + (dd ? 24 : 0);
+                const u64 free_b = j.cap - j.top;
                 u64 cap;
-                if (!f.chunking && f.n_next * kTNode + 8 * kAl <= free_b / 2 + free_b / 4) {
+                if (!f.chunking && f.n_next * node_b + 9 * kAl <= free_b / 2 + free_b / 4) {
                     cap = f.n_next;
                 } else {
                     if (!f.chunking) {
                         f.chunking = true;
                         f.end = f.off + f.m;
                     }
-                    const u64 per = 218 * kTNode;
+                    const u64 per = 218 * node_b;
                     const u64 pm = std::max<u64>(1, (free_b / 2) / per);
                     f.m = std::min(pm, f.end - f.off);
                     a.n_in = (u32)f.m;
@@ -236,6 +358,15 @@ bool Server::plan(Job& j) {
                 if (!alloc_level(j, cap, &f.nx)) {
                     j.top = f.mark;
                     return false;  // cannot happen with the half-free rule; wait
+                }
+                f.level_top = j.top;
+                if (dd && cap >= 2) {
+                    const u64 dn = dup_table_slots(cap);
+                    DupEntry* dup = reinterpret_cast<DupEntry*>(take(j, dn * sizeof(DupEntry)));
+                    if (dup) {  // without room the level simply keeps its duplicates
+                        a.dup = dup;
+                        a.dup_n = dn;
+                    }
                 }
                 f.cap = cap;
                 a.mode = kWrite;
@@ -256,17 +387,9 @@ bool Server::plan(Job& j) {
             return true;
         }
         if (f.state == ST_CHILD) {
-            if (t.nup >= (u32)kMaxUps) return false;
-            const u32 ui = t.nup++;
-            UpSeg& u = t.up[ui];
-            u.parent = f.nx.parent;
-            u.k1 = f.nx.k1;
-            u.k2 = f.nx.k2;
-            u.acc = f.nx.acc;
-            u.parent_acc = f.lv.acc;
-            u.n = (u32)f.written;
-            u.depth = r - 1;
-            f.state = ST_UP;
+            const int n = chain_up(j, (int)j.st.size() - 1);
+            if (n == 0) return false;  // no upsweep room left in this step
+            j.chain = n;
             j.pending = 3;
             return true;
         }
@@ -296,15 +419,29 @@ void Server::run() {
             std::unique_lock<std::mutex> g(mu_);
             cv_.wait(g, [&] { return stop_ || !queue_.empty() || !active.empty(); });
             if (stop_ && queue_.empty() && active.empty()) break;
-            // admission: one slot (an equal share of the frontier arena) per job
-            while (!queue_.empty() && !free_slots_.empty() && (int)admitted.size() < kMaxRoots) {
-                Job jb = std::move(queue_.front());
-                queue_.pop_front();
-                if (jb.depth == 0) {
-                    pf_result res{jb.tag, 1, PF_OK, 0, jb.t_submit, now_s()};
+            // admission: a free slot and a free region of the predicted size
+            const double tnow = now_s();
+            size_t scanned = 0;
+            for (auto it = queue_.begin(); it != queue_.end() && scanned < 4096;) {
+                if (free_slots_.empty() || (int)admitted.size() >= kMaxRoots) break;
+                scanned++;
+                if (it->depth == 0) {
+                    pf_result res{it->tag, 1, PF_OK, 0, it->t_submit, tnow};
                     done_.push_back(res);
+                    it = queue_.erase(it);
                     continue;
                 }
+                u64 off = 0;
+                if (!region_alloc(it->want, &off)) {
+                    // an old request that does not fit stops younger ones from passing it
+                    if (tnow - it->t_submit > 0.050) break;
+                    ++it;
+                    continue;
+                }
+                Job jb = std::move(*it);
+                it = queue_.erase(it);
+                jb.base = off;
+                jb.cap = jb.want;
                 const int slot = free_slots_.back();
                 free_slots_.pop_back();
                 jb.slot = slot;
@@ -315,6 +452,15 @@ void Server::run() {
         StepTable& t = *h_tab_;
         t.nseg = t.nup = t.nroot = 0;
         t.tile_ctr = 0;
+        for (auto& v : up_) v.clear();
+        nup_planned_ = 0;
+        // running jobs first, then the new ones: each new root is probed and
+        // counted by the start kernel, and its first level is expanded by the
+        // step kernel of the same step
+        for (int slot : active) {
+            Job& j = jobs_[(size_t)slot];
+            if (j.pending == 0 && !j.done) plan(j);
+        }
         for (int slot : admitted) {
             Job& j = jobs_[(size_t)slot];
             const int black = pos_black_to_move(j.pos);
@@ -329,18 +475,31 @@ void Server::run() {
             rs.dst_k2 = root_k2_ + slot;
             rs.dst_w = root_w_ + slot;
             rs.acc = slot_acc_ + slot;
-            j.pending = 1;
             j.top = 0;
             j.st.clear();
-        }
-        for (int slot : active) {
-            Job& j = jobs_[(size_t)slot];
-            if (j.pending == 0 && !j.done) plan(j);
+            j.pending = 1;
+            if (j.depth >= 2) {
+                Frame f = {};
+                f.lv.b = root_b_ + slot;
+                f.lv.root = root_w_ + slot;
+                f.lv.k1 = root_k1_ + slot;
+                f.lv.k2 = root_k2_ + slot;
+                f.lv.parent = root_p_ + slot;
+                f.lv.acc = slot_acc_ + slot;
+                f.off = 0;
+                f.m = 1;
+                f.n_next = (u64)count_moves(rs.b, KSBackend{});  // what the start kernel will store
+                f.L = 0;
+                f.state = ST_START;
+                j.st.push_back(f);
+                if (!plan(j)) j.pending = 4;  // root only this step; planned with the running jobs next
+            }
+            active.push_back(slot);
         }
         // drop finished jobs from the active list
         active.erase(std::remove_if(active.begin(), active.end(), [&](int s) { return jobs_[(size_t)s].done; }),
                      active.end());
-        if (t.nseg == 0 && t.nup == 0 && t.nroot == 0) {
+        if (t.nseg == 0 && nup_planned_ == 0 && t.nroot == 0) {
             // nothing can move: a job waiting for memory inside its own region
             // will never get it, so fail such jobs instead of spinning
             for (int slot : active)
@@ -355,11 +514,18 @@ void Server::run() {
             acc_t += (t.seg[i].n_in + 127) / 128;
         }
         t.tile_prefix[t.nseg] = acc_t;
+        // upsweep segments laid out round by round
         u32 acc_u = 0;
-        for (u32 i = 0; i < t.nup; i++) {
-            t.up_prefix[i] = acc_u;
-            acc_u += t.up[i].n;
+        for (int r = 0; r < kUpRounds; r++) {
+            t.up_round[r] = t.nup;
+            for (const UpSeg& u : up_[(size_t)r]) {
+                t.up[t.nup] = u;
+                t.up_prefix[t.nup] = acc_u;
+                acc_u += u.n;
+                t.nup++;
+            }
         }
+        t.up_round[kUpRounds] = t.nup;
         t.up_prefix[t.nup] = acc_u;
         memset(t.out, 0, sizeof(SegOut) * t.nseg);
         const double t0 = now_s();
@@ -372,66 +538,62 @@ void Server::run() {
             busy_ += t1 - t0;
         }
         if (!ok) {
-            for (int slot : active) finish(jobs_[(size_t)slot], 0, PF_EDEVICE);
-            for (int slot : admitted) finish(jobs_[(size_t)slot], 0, PF_EDEVICE);
+            for (int slot : active)
+                if (!jobs_[(size_t)slot].done) finish(jobs_[(size_t)slot], 0, PF_EDEVICE);
             active.clear();
             continue;
         }
         // outcomes
-        for (size_t i = 0; i < admitted.size(); i++) {
-            Job& j = jobs_[(size_t)admitted[i]];
-            j.pending = 0;
-            const RootOut& ro = t.rout[i];
-            if (ro.hit) { finish(j, h_acc_[j.slot], PF_OK); continue; }
-            if (j.depth == 1) { finish(j, ro.count, PF_OK); continue; }
-            Frame f = {};
-            f.lv.b = root_b_ + j.slot;
-            f.lv.root = root_w_ + j.slot;
-            f.lv.k1 = root_k1_ + j.slot;
-            f.lv.k2 = root_k2_ + j.slot;
-            f.lv.parent = root_p_ + j.slot;
-            f.lv.acc = slot_acc_ + j.slot;
-            f.off = 0;
-            f.m = 1;
-            f.n_next = ro.count;
-            f.L = 0;
-            f.state = ST_START;
-            j.st.push_back(f);
-            active.push_back(j.slot);
-        }
+        u64 step_dups = 0;
         for (int slot : active) {
             Job& j = jobs_[(size_t)slot];
-            if (j.done || j.st.empty()) continue;
+            if (j.done) continue;
+            const int pend = j.pending;
+            j.pending = 0;
+            if (pend == 1 || pend == 4) {  // a new root alone this step
+                if (j.depth == 1) finish(j, h_acc_[j.slot], PF_OK);
+                continue;
+            }
+            if (pend == 3) {  // upsweeps alone
+                if (after_chain(j, j.chain)) finish(j, h_acc_[j.slot], PF_OK);
+                continue;
+            }
+            if (pend != 2) continue;
+            const SegOut& o = t.out[j.seg];
+            if (o.err) {
+                finish(j, 0, PF_ENOMEM);
+                continue;
+            }
+            step_dups += o.dups;
             Frame& f = j.st.back();
-            if (j.pending == 2) {
-                const SegOut& o = t.out[j.seg];
-                j.pending = 0;
-                if (o.err) { finish(j, 0, PF_ENOMEM); continue; }
-                if (f.state == ST_LEAF) {
-                    j.st.pop_back();
-                    if (!j.st.empty()) j.st.back().state = ST_CHILD;
-                } else if (o.out_count == 0) {
-                    f.written = 0;
-                    f.state = ST_UPDONE;
-                } else {  // descend into the written children
-                    f.written = o.out_count;
-                    f.state = ST_WAITCHILD;
-                    Frame c = {};
-                    c.lv = f.nx;
-                    c.off = 0;
-                    c.m = o.out_count;
-                    c.n_next = o.total;
-                    c.L = f.L + 1;
-                    c.state = ST_START;
-                    j.st.push_back(c);
-                }
-            } else if (j.pending == 3) {
-                j.pending = 0;
+            if (f.state == ST_LEAF) {
+                j.st.pop_back();
+                if (j.st.empty() || after_chain(j, j.chain)) finish(j, h_acc_[j.slot], PF_OK);
+                continue;
+            }
+            j.top = f.level_top;  // the duplicate table has served
+            if (o.out_count == 0) {
+                f.written = 0;
                 f.state = ST_UPDONE;
+            } else {  // descend into the written children
+                f.written = o.out_count;
+                f.state = ST_WAITCHILD;
+                Frame c = {};
+                c.lv = f.nx;
+                c.off = 0;
+                c.m = o.out_count;
+                c.n_next = o.total - o.removed;
+                c.L = f.L + 1;
+                c.state = ST_START;
+                j.st.push_back(c);
             }
         }
         active.erase(std::remove_if(active.begin(), active.end(), [&](int s) { return jobs_[(size_t)s].done; }),
                      active.end());
+        if (step_dups) {
+            std::lock_guard<std::mutex> g(mu_);
+            dups_ += step_dups;
+        }
     }
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -20,7 +21,7 @@ class Server {
     void stop();
     pf_status submit(const pf_pos48& pos, int depth, u64 tag);
     int poll(pf_result* out, int max);
-    void counters(u64* steps, u64* segs, double* busy_s);
+    void counters(u64* steps, u64* segs, double* busy_s, u64* dups);
 
   private:
     struct Frame {
@@ -31,6 +32,7 @@ class Server {
         bool chunking;
         Engine::TLevel nx;
         u64 cap, mark, written;
+        u64 level_top;  // stack top just above the children's level
     };
     struct Job {
         u64 tag = 0;
@@ -38,16 +40,25 @@ class Server {
         double t_submit = 0;
         int slot = -1;
         pf_pos48 pos{};
+        u64 base = 0, cap = 0;  // the job's arena region [base, base + cap)
         u64 top = 0;  // stack top within the job's arena region
+        u64 want = 0;  // predicted frontier bytes (the region it asks for)
         std::vector<Frame> st;
-        int pending = 0;  // 0 none, 1 root, 2 expand, 3 upsweep
+        int pending = 0;  // 0 none, 1/4 new root only, 2 expand (+ upsweep chain), 3 upsweep chain
+        int chain = 0;    // upsweeps chained in the pending action
         int seg = -1;
         bool done = false;
     };
 
     void run();
+    u64 predict_bytes(const pf_pos48& pos, int depth) const;
+    bool region_alloc(u64 bytes, u64* off);
+    void region_free(u64 off, u64 bytes);
     bool plan(Job& j);
+    int chain_up(Job& j, int k);
+    bool after_chain(Job& j, int n);
     void finish(Job& j, u64 nodes, int status);
+    char* take(Job& j, u64 bytes);
     bool alloc_level(Job& j, u64 n, Engine::TLevel* lv);
     pf_status build_graph();
 
@@ -62,6 +73,10 @@ class Server {
     std::vector<pf_result> done_;  // finished, not yet polled
     std::vector<Job> jobs_;        // indexed by slot
     std::vector<int> free_slots_;
+    std::vector<UpSeg> up_[kUpRounds];  // this step's upsweeps, by round
+    u32 nup_planned_ = 0;
+    std::map<u64, u64> free_mem_;  // arena offset -> bytes of each free block (coalesced)
+    u64 max_region_ = 0;
     // device / pinned resources
     StepTable* h_tab_ = nullptr;
     StepTable* d_tab_ = nullptr;
@@ -75,7 +90,7 @@ class Server {
     int grid_ = 0, up_grid_ = 0;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
-    u64 steps_ = 0, segs_ = 0;
+    u64 steps_ = 0, segs_ = 0, dups_ = 0;
     double busy_ = 0;
 };
 

@@ -299,32 +299,40 @@ __global__ __launch_bounds__(256) void k_roots_tt(const SliderTables* tabs, Boar
 // records the original, whose exact count it takes in the upsweep
 // (perft_bb.h:871-878).  removed[0] += the duplicates' move counts,
 // removed[1] += the number of duplicates.
+// Node i of a level of n nodes: returns its move count if it is a duplicate
+// (and marks it), else 0 with *dup = false.
+__device__ __forceinline__ u32 dedup_node(Board* b, const u64* k1, const u64* k2, unsigned long long* acc, u32 i,
+                                          u32 n, const DupEntry* t, u64 tn, bool* dup) {
+    *dup = false;
+    const u64 key2 = k2[i];
+    if (!key2) return 0;
+    const u64 key1 = k1[i];
+    u64 s = dup_slot(key1, tn);
+    for (int p = 0; p < kDupProbes; p++) {
+        const DupEntry e = t[s];
+        if (e.key == 0) return 0;
+        if (e.key == key2) {
+            const u32 o = (u32)e.val;
+            if ((e.val >> 32) != (key1 >> 32) || o == i || o >= n) return 0;
+            const u64 pw = b[i].pawns;
+            b[i].pawns = pw & ~(0xFFull << 56);
+            acc[i] = kDupBit | o;
+            *dup = true;
+            return (u32)(pw >> 56);
+        }
+        s = (s + 1 == tn) ? 0 : s + 1;
+    }
+    return 0;
+}
+
 __global__ __launch_bounds__(256) void k_dedup(Board* b, const u64* k1, const u64* k2, unsigned long long* acc, u32 n,
                                                const DupEntry* t, u64 tn, unsigned long long* removed) {
     const u32 i = blockIdx.x * 256 + threadIdx.x;
     unsigned long long mv = 0, nd = 0;
     if (i < n) {
-        const u64 key2 = k2[i];
-        if (key2) {
-            const u64 key1 = k1[i];
-            u64 s = dup_slot(key1, tn);
-            for (int p = 0; p < kDupProbes; p++) {
-                const DupEntry e = t[s];
-                if (e.key == 0) break;
-                if (e.key == key2) {
-                    const u32 o = (u32)e.val;
-                    if ((e.val >> 32) == (key1 >> 32) && o != i && o < n) {
-                        const u64 pw = b[i].pawns;
-                        mv = pw >> 56;
-                        b[i].pawns = pw & ~(0xFFull << 56);
-                        acc[i] = kDupBit | o;
-                        nd = 1;
-                    }
-                    break;
-                }
-                s = (s + 1 == tn) ? 0 : s + 1;
-            }
-        }
+        bool d = false;
+        mv = dedup_node(b, k1, k2, acc, i, n, t, tn, &d);
+        nd = d ? 1 : 0;
     }
     mv = t_wave_sum64(mv);
     nd = t_wave_sum64(nd);
@@ -396,50 +404,93 @@ __global__ __launch_bounds__(kTWG) void k_step(const SliderTables* tabs, const Z
     }
 }
 
-// Upsweep segments: thread i of the concatenation finds its segment by search.
-__global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 mask, unsigned long long* stores) {
-    const u32 nup = t->nup, total = t->up_prefix[nup];
-    for (u32 i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256) {
-        u32 lo = 0, hi = nup;
+// Duplicates among the children that this step's write segments wrote
+// (k_dedup per segment); the segment's next-level total loses their moves.
+__device__ void dedup_step_segments(StepTable* t) {
+    const u32 stride = gridDim.x * 256;
+    for (u32 si = 0; si < t->nseg; si++) {
+        const TTArgs& a = t->seg[si];
+        if (!a.dup) continue;
+        // children past the capacity were not written (the segment reports an error)
+        const u32 n = (u32)min((u64)t->out[si].out_count, a.out_cap);
+        unsigned long long mv = 0, nd = 0;
+        // every lane runs the same trip count, so the wave sums below are uniform
+        for (u32 base = blockIdx.x * 256; base < n; base += stride) {
+            const u32 i = base + threadIdx.x;
+            if (i < n) {
+                bool d = false;
+                mv += dedup_node(a.out, a.out_k1, a.out_k2, a.out_acc, i, n, a.dup, a.dup_n, &d);
+                nd += d ? 1 : 0;
+            }
+        }
+        mv = t_wave_sum64(mv);
+        nd = t_wave_sum64(nd);
+        if ((threadIdx.x & 63) == 0 && nd) {
+            atomicAdd(&t->out[si].removed, mv);
+            atomicAdd(&t->out[si].dups, nd);
+        }
+    }
+}
+
+// Upsweep round `round`: thread i of the round's concatenated segments finds
+// its segment by search.  Round 0 first runs the duplicate pass over the
+// levels written in this step (no upsweep of this step reads them).
+__global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 mask, int round) {
+    if (round == 0) dedup_step_segments(t);
+    const u32 s0 = t->up_round[round], s1 = t->up_round[round + 1];
+    const u32 first = t->up_prefix[s0], last = t->up_prefix[s1];
+    for (u32 i = first + blockIdx.x * 256 + threadIdx.x; i < last; i += gridDim.x * 256) {
+        u32 lo = s0, hi = s1;  // largest segment with up_prefix[seg] <= i
         while (hi - lo > 1) {
             const u32 mid = (lo + hi) >> 1;
             if (t->up_prefix[mid] <= i) lo = mid; else hi = mid;
         }
         const UpSeg& u = t->up[lo];
         const u32 k = i - t->up_prefix[lo];
-        const unsigned long long v = u.acc[k];
+        unsigned long long v = u.acc[k];
+        const bool dup = (v & kDupBit) != 0;
+        if (dup) v = u.acc[(u32)v];
         if (u.parent_acc) atomicAdd(&u.parent_acc[u.parent[k]], v);
-        if (tt) tt_store(tt, mask, u.k1[k], u.k2[k], u.depth, v);
+        if (tt && !dup) tt_store(tt, mask, u.k1[k], u.k2[k], u.depth, v);
     }
-    (void)stores;
 }
 
-// New requests: probe the table at full depth, count the root's moves.
+// Start of a step: new requests (table probe at full depth, move count) and
+// empty duplicate tables for this step's write segments (a fill kernel: the
+// step is a captured graph, which records no memset).
 template <int B>
-__global__ __launch_bounds__(64) void k_roots_step(const SliderTables* tabs, StepTable* t, const uint4* tt, u64 mask) {
+__global__ __launch_bounds__(256) void k_pre_step(const SliderTables* tabs, StepTable* t, const uint4* tt, u64 mask) {
     const auto sl = SlOfT<B>::make(tabs);
-    const u32 i = blockIdx.x * 64 + threadIdx.x;
-    if (i >= t->nroot) return;  // roots are few: a 64-thread block each 64
-    const RootSeg& r = t->root[i];
-    const Board b = r.b;
-    *r.dst_w = r.word;
-    if (tt) {
-        *r.dst_k1 = r.k1;
-        *r.dst_k2 = r.k2;
+    const u32 gi = blockIdx.x * 256 + threadIdx.x;
+    if (gi < t->nroot) {
+        const RootSeg& r = t->root[gi];
+        const Board b = r.b;
+        *r.dst_w = r.word;
+        if (tt) {
+            *r.dst_k1 = r.k1;
+            *r.dst_k2 = r.k2;
+        }
+        u64 v = 0;
+        if (tt && r.depth >= 2 && tt_probe(tt, mask, r.k1, r.k2, r.depth, &v)) {
+            *r.acc = v;
+            *r.dst_b = with_count(b, 0);  // answered: nothing below it is expanded
+            t->rout[gi].hit = 1;
+            t->rout[gi].count = 0;
+        } else {
+            const u32 c = count_moves(b, sl);
+            *r.dst_b = with_count(b, c);
+            *r.acc = (r.depth == 1) ? c : 0;
+            t->rout[gi].hit = 0;
+            t->rout[gi].count = c;
+        }
     }
-    u64 v = 0;
-    if (tt && r.depth >= 2 && tt_probe(tt, mask, r.k1, r.k2, r.depth, &v)) {
-        *r.acc = v;
-        *r.dst_b = with_count(b, 0);
-        t->rout[i].hit = 1;
-        t->rout[i].count = 0;
-        return;
+    const u64 stride = (u64)gridDim.x * 256;
+    for (u32 si = 0; si < t->nseg; si++) {
+        DupEntry* d = t->seg[si].dup;
+        if (!d) continue;
+        const u64 n = t->seg[si].dup_n;
+        for (u64 i = gi; i < n; i += stride) d[i] = DupEntry{0, 0};
     }
-    const u32 c = count_moves(b, sl);
-    *r.dst_b = with_count(b, c);
-    *r.acc = (r.depth == 1) ? c : 0;
-    t->rout[i].hit = 0;
-    t->rout[i].count = c;
 }
 
 int step_grid(int backend, int n_cu) {
@@ -450,19 +501,23 @@ int step_grid(int backend, int n_cu) {
     return per_cu * n_cu;
 }
 
+// One step: start (roots, fills), expand/leaf segments, kUpRounds upsweep
+// rounds.  up_grid * 256 must cover kMaxRoots (the roots are handled by the
+// first threads of the start kernel).
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
                        int grid, int up_grid, hipStream_t s) {
+    if (up_grid * 256 < kMaxRoots) return hipErrorInvalidValue;
     switch (backend) {
-    case kKS: hipLaunchKernelGGL(k_roots_step<kKS>, dim3(kMaxRoots / 64), dim3(64), 0, s, tabs, t, tt, mask); break;
-    case kPlain: hipLaunchKernelGGL(k_roots_step<kPlain>, dim3(kMaxRoots / 64), dim3(64), 0, s, tabs, t, tt, mask); break;
-    default: hipLaunchKernelGGL(k_roots_step<kFancy>, dim3(kMaxRoots / 64), dim3(64), 0, s, tabs, t, tt, mask); break;
+    case kKS: hipLaunchKernelGGL(k_pre_step<kKS>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask); break;
+    case kPlain: hipLaunchKernelGGL(k_pre_step<kPlain>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask); break;
+    default: hipLaunchKernelGGL(k_pre_step<kFancy>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask); break;
     }
     switch (backend) {
     case kKS: hipLaunchKernelGGL(k_step<kKS>, dim3(grid), dim3(kTWG), 0, s, tabs, zt, t); break;
     case kPlain: hipLaunchKernelGGL(k_step<kPlain>, dim3(grid), dim3(kTWG), 0, s, tabs, zt, t); break;
     default: hipLaunchKernelGGL(k_step<kFancy>, dim3(grid), dim3(kTWG), 0, s, tabs, zt, t); break;
     }
-    hipLaunchKernelGGL(k_up_step, dim3(up_grid), dim3(256), 0, s, t, tt, mask, nullptr);
+    for (int r = 0; r < kUpRounds; r++) hipLaunchKernelGGL(k_up_step, dim3(up_grid), dim3(256), 0, s, t, tt, mask, r);
     return hipGetLastError();
 }
 

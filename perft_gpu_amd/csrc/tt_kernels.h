@@ -20,6 +20,14 @@ struct alignas(16) DupEntry {
 };
 constexpr int kDupProbes = 32;
 constexpr unsigned long long kDupBit = 1ULL << 63;  // acc of a duplicate: kDupBit | original's index
+constexpr int kDupMinPly = 3;  // no two move sequences shorter than 3 plies reach the same position
+
+// Slots of the duplicate table of a level of n nodes: linear probing at a load
+// of at most 2/3, indices below 2^32.
+inline u64 dup_table_slots(u64 n) {
+    const u64 s = n + n / 2 + 64;
+    return s > 0xFFFFFFFFull ? 0xFFFFFFFFull : s;
+}
 
 struct TTArgs {
     // parents: a contiguous range of one level
@@ -82,15 +90,23 @@ struct SegOut {
     u32 out_count;
     u32 err;
     unsigned long long total;
+    unsigned long long removed;  // moves of the children found to be duplicates
+    unsigned long long dups;     // children found to be duplicates
 };
 struct RootOut {
     u32 hit;
     u32 count;
 };
+// Upsweeps of one step run in kUpRounds dependent launches: round r+1 adds
+// levels whose own children were added in round r (a finished leaf level and
+// the chain of levels above it complete in one step).
+constexpr int kUpRounds = 4;
+
 struct StepTable {
     u32 nseg, nup, nroot, tile_ctr;
     u32 tile_prefix[kMaxSegs + 1];
-    u32 up_prefix[kMaxUps + 1];
+    u32 up_prefix[kMaxUps + 1];    // node prefix over all upsweep segments
+    u32 up_round[kUpRounds + 1];   // segments [up_round[r], up_round[r+1]) belong to round r
     TTArgs seg[kMaxSegs];
     UpSeg up[kMaxUps];
     RootSeg root[kMaxRoots];

@@ -49,6 +49,8 @@ def test_serving_alone_equals_mixed(pg, oracle, tt):
             assert got[i]["status"] == 0 and got[i]["nodes"] == want[i], (i, depths[i])
             assert got[100 + i]["nodes"] == want[i]
             assert got[i]["done_s"] >= got[i]["submit_s"]
+        if tt:  # depth-5 requests write a level at ply 3: duplicates were found and merged
+            assert eng.stats()["dup_nodes"] > 0
         eng.serve_stop()
         # the context is usable for plain perft again after serving
         assert eng.perft(pg.fen_to_pos(pg.START_FEN), 4) == 197281
