@@ -61,6 +61,7 @@ struct Engine {
         u64* k2;
         u32* parent;
         unsigned long long* acc;
+        u32* bsum;  // per 1024-node block: sum of the nodes' move counts (serving levels only)
     };
 
     pf_status init(const pf_config& cfg);

@@ -56,6 +56,7 @@ bool Engine::alloc_tlevel(u64 n, TLevel* lv) {
     lv->k2 = reinterpret_cast<u64*>(alloc(n * 8));
     lv->parent = reinterpret_cast<u32*>(alloc(n * 4));
     lv->acc = reinterpret_cast<unsigned long long*>(alloc(n * 8));
+    lv->bsum = nullptr;
     return lv->b && lv->root && lv->k1 && lv->k2 && lv->parent && lv->acc;
 }
 

@@ -245,6 +245,41 @@ bool Server::alloc_level(Job& j, u64 n, Engine::TLevel* lv) {
     return lv->b && lv->root && lv->k1 && lv->k2 && lv->parent && lv->acc;
 }
 
+// Next chunk of a chunking frame f: parents [f.off, f.off + f.m) whose
+// children fit in `budget` nodes; *cap = the children they can have.  From a
+// block boundary it takes whole 1024-parent blocks by their exact move sums
+// (the level's block sums, copied to the host when the level first chunks);
+// inside a block, or when one block alone is too big, as many parents as fit
+// at 218 children each, up to the next block boundary.
+bool Server::chunk(Job& j, Frame& f, u64 budget, u64* cap) {
+    (void)j;
+    if (f.lv.bsum && (f.off & (kBsumBlock - 1)) == 0) {
+        const u64 nblk = (f.end + kBsumBlock - 1) >> kBsumShift;
+        if (f.hbsum.size() != nblk) {
+            f.hbsum.assign((size_t)nblk, 0);
+            if (hipMemcpyAsync(f.hbsum.data(), f.lv.bsum, nblk * 4, hipMemcpyDeviceToHost, e_.stream) != hipSuccess ||
+                hipStreamSynchronize(e_.stream) != hipSuccess)
+                return false;
+        }
+        u64 m = 0, sum = 0;
+        for (u64 b = f.off >> kBsumShift; b < nblk; b++) {
+            if (sum + f.hbsum[(size_t)b] > budget) break;
+            sum += f.hbsum[(size_t)b];
+            m += std::min<u64>(kBsumBlock, f.end - (b << kBsumShift));
+        }
+        if (m) {
+            f.m = m;
+            *cap = sum;
+            return true;
+        }
+    }
+    u64 lim = f.end - f.off;
+    if (f.lv.bsum) lim = std::min(lim, (((f.off >> kBsumShift) + 1) << kBsumShift) - f.off);
+    f.m = std::min(std::max<u64>(1, budget / 218), lim);
+    *cap = f.m * 218;
+    return true;
+}
+
 // Frames st[k], st[k-1], ... have complete child levels: schedule their
 // upsweeps in rounds 0, 1, ... of this step, up to a frame that still has
 // chunks to run (its next chunk reuses the memory being upswept, so it waits
@@ -341,23 +376,25 @@ bool Server::plan(Job& j) {
                 const u64 node_b = kTNode + (dd ? 24 : 0);
                 const u64 free_b = j.cap - j.top;
                 u64 cap;
-                if (!f.chunking && f.n_next * node_b + 9 * kAl <= free_b / 2 + free_b / 4) {
+                if (!f.chunking && f.n_next * node_b + 10 * kAl <= free_b / 2 + free_b / 4) {
                     cap = f.n_next;
                 } else {
                     if (!f.chunking) {
                         f.chunking = true;
                         f.end = f.off + f.m;
                     }
-                    const u64 per = 218 * node_b;
-                    const u64 pm = std::max<u64>(1, (free_b / 2) / per);
-                    f.m = std::min(pm, f.end - f.off);
+                    if (!chunk(j, f, (free_b / 2) / node_b, &cap)) return false;
                     a.n_in = (u32)f.m;
-                    cap = f.m * 218;
                 }
                 f.mark = j.top;
                 if (!alloc_level(j, cap, &f.nx)) {
                     j.top = f.mark;
                     return false;  // cannot happen with the half-free rule; wait
+                }
+                f.nx.bsum = reinterpret_cast<u32*>(take(j, ((cap + kBsumBlock - 1) >> kBsumShift) * 4));
+                if (!f.nx.bsum) {
+                    j.top = f.mark;
+                    return false;
                 }
                 f.level_top = j.top;
                 if (dd && cap >= 2) {
@@ -377,6 +414,8 @@ bool Server::plan(Job& j) {
                 a.out_parent = f.nx.parent;
                 a.out_acc = f.nx.acc;
                 a.out_cap = cap;
+                a.out_bsum = f.nx.bsum;
+                a.bsum_n = (u32)((cap + kBsumBlock - 1) >> kBsumShift);
                 a.total = &d_tab_->out[si].total;
                 f.state = ST_WRITE;
             }

@@ -33,6 +33,7 @@ class Server {
         Engine::TLevel nx;
         u64 cap, mark, written;
         u64 level_top;  // stack top just above the children's level
+        std::vector<u32> hbsum;  // host copy of lv.bsum, fetched when the level first chunks
     };
     struct Job {
         u64 tag = 0;
@@ -56,6 +57,7 @@ class Server {
     void region_free(u64 off, u64 bytes);
     bool plan(Job& j);
     int chain_up(Job& j, int k);
+    bool chunk(Job& j, Frame& f, u64 budget, u64* cap);
     bool after_chain(Job& j, int n);
     void finish(Job& j, u64 nodes, int status);
     char* take(Job& j, u64 bytes);

@@ -220,6 +220,17 @@ __device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTa
                             atomicOr(a.err, 1u);
                         }
                     }
+                    if (a.out_bsum && nk) {
+                        // the wave's children are consecutive: at most two blocks
+                        const u64 b0 = (u64)base >> kBsumShift;
+                        const bool in = keep && pos < a.out_cap;
+                        const u64 part = in ? (((pos >> kBsumShift) == b0) ? (u64)cc : (u64)cc << 32) : 0;
+                        const u64 tot = t_wave_sum64(part);
+                        if (lane == 0) {
+                            if ((u32)tot) atomicAdd(&a.out_bsum[b0], (u32)tot);
+                            if (tot >> 32) atomicAdd(&a.out_bsum[b0 + 1], (u32)(tot >> 32));
+                        }
+                    }
                 }
             }
             __syncthreads();
@@ -417,10 +428,16 @@ __device__ void dedup_step_segments(StepTable* t) {
         // every lane runs the same trip count, so the wave sums below are uniform
         for (u32 base = blockIdx.x * 256; base < n; base += stride) {
             const u32 i = base + threadIdx.x;
+            u32 m = 0;
             if (i < n) {
                 bool d = false;
-                mv += dedup_node(a.out, a.out_k1, a.out_k2, a.out_acc, i, n, a.dup, a.dup_n, &d);
+                m = dedup_node(a.out, a.out_k1, a.out_k2, a.out_acc, i, n, a.dup, a.dup_n, &d);
                 nd += d ? 1 : 0;
+            }
+            mv += m;
+            if (a.out_bsum) {  // a wave's 64 nodes lie in one block
+                const unsigned long long wm = t_wave_sum64(m);
+                if ((threadIdx.x & 63) == 0 && wm) atomicSub(&a.out_bsum[i >> kBsumShift], (u32)wm);
             }
         }
         mv = t_wave_sum64(mv);
@@ -486,10 +503,11 @@ __global__ __launch_bounds__(256) void k_pre_step(const SliderTables* tabs, Step
     }
     const u64 stride = (u64)gridDim.x * 256;
     for (u32 si = 0; si < t->nseg; si++) {
-        DupEntry* d = t->seg[si].dup;
-        if (!d) continue;
-        const u64 n = t->seg[si].dup_n;
-        for (u64 i = gi; i < n; i += stride) d[i] = DupEntry{0, 0};
+        const TTArgs& a = t->seg[si];
+        if (a.out_bsum)
+            for (u64 i = gi; i < a.bsum_n; i += stride) a.out_bsum[i] = 0;
+        if (!a.dup) continue;
+        for (u64 i = gi; i < a.dup_n; i += stride) a.dup[i] = DupEntry{0, 0};
     }
 }
 

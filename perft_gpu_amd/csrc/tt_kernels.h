@@ -21,6 +21,8 @@ struct alignas(16) DupEntry {
 constexpr int kDupProbes = 32;
 constexpr unsigned long long kDupBit = 1ULL << 63;  // acc of a duplicate: kDupBit | original's index
 constexpr int kDupMinPly = 3;  // no two move sequences shorter than 3 plies reach the same position
+constexpr u32 kBsumShift = 10;  // block sums of move counts over 1024 consecutive nodes
+constexpr u32 kBsumBlock = 1u << kBsumShift;
 
 // Slots of the duplicate table of a level of n nodes: linear probing at a load
 // of at most 2/3, indices below 2^32.
@@ -57,6 +59,8 @@ struct TTArgs {
     unsigned long long* stats;   // [0] probes, [1] hits
     DupEntry* dup;               // written children claim a slot here (null: no duplicate detection)
     u64 dup_n;                   // slots (< 2^32)
+    u32* out_bsum;               // per kBsumBlock children: sum of their move counts (null: not kept)
+    u32 bsum_n;                  // blocks (cleared by the serving step's start kernel)
 };
 
 // One serving step (serve.cpp): expand/leaf segments of many jobs, their

@@ -97,3 +97,21 @@ def test_serving_chunked_small_slots(pg, oracle):
         eng.serve_stop()
     finally:
         eng.close()
+
+
+def test_serving_chunks_by_block_sums(pg, oracle):
+    """16 MiB per job: the ply-4 levels of depth-6 requests do not fit, and the
+    jobs chunk their ply-3 parents by exact block sums of their move counts"""
+    eng = pg.Engine(backend="ks", arena_bytes=64 << 20, tt_bytes=64 << 20, zobrist_seed=9)
+    try:
+        eng.serve_start(max_jobs=4, job_bytes=16 << 20)
+        pos = pg.playouts(2024, 4, 12, 30)
+        want = oracle.perft_batch(pos, 6, nthreads=8)
+        for i in range(4):
+            eng.submit(pos[i], 6, i)
+        got = _drain(eng, 4, timeout=300.0)
+        assert [got[i]["nodes"] for i in range(4)] == [int(x) for x in want]
+        assert all(got[i]["status"] == 0 for i in range(4))
+        eng.serve_stop()
+    finally:
+        eng.close()
