@@ -148,8 +148,10 @@ def run_plain_requests(pg, W, seconds, backend):
             by_depth[str(d)] = by_depth.get(str(d), 0.0) + time.perf_counter() - t1
         el = time.perf_counter() - t
         nodes = sum(counts)
+        st = eng.stats()
         return {"requests": len(reqs), "nodes": nodes, "seconds": el, "nodes_per_s": nodes / el,
-                "seconds_by_depth": by_depth, "_counts": counts}
+                "seconds_by_depth": by_depth, "dup_nodes": st["dup_nodes"], "expand_gpu_s": st["gpu_ms"] * 1e-3,
+                "_counts": counts}
     finally:
         eng.close()
 

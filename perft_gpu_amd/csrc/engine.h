@@ -62,6 +62,12 @@ struct Engine {
         u32* parent;
         unsigned long long* acc;
         u32* bsum;  // per 1024-node block: sum of the nodes' move counts (serving levels only)
+        // duplicates of the level's nodes: claim table and list (null: none)
+        DupEntry* dup;
+        u64 dup_n;
+        u32* dpar;
+        u32* dslot;
+        u64 ndup;
     };
 
     pf_status init(const pf_config& cfg);
@@ -82,9 +88,8 @@ struct Engine {
     pf_status upload_roots(const pf_pos48* pos, int n, Board** roots, u32** rid);
     pf_status descend(const Board* lvl, const u32* rts, u64 n, u64 n_next, int L);
     bool alloc_tlevel(u64 n, TLevel* lv);
-    pf_status expand_tt(int mode, const TLevel& lv, u64 off, u64 m, const TLevel* nx, u64 cap, int L,
-                        u64* written, u64* next_total, DupEntry* dup = nullptr, u64 dup_n = 0);
-    pf_status dedup_level(const TLevel& lv, u64 n, DupEntry* dup, u64 dup_n, u64* removed_moves);
+    pf_status expand_tt(int mode, const TLevel& lv, u64 off, u64 m, TLevel* nx, u64 cap, int L, u64* written,
+                        u64* next_total);
     pf_status descend_tt(const TLevel& lv, u64 off, u64 m, u64 n_next, int L);
   public:
     pf_status perft_levels_tt(const pf_pos48* pos, int n, int depth, u64* out);

@@ -1,14 +1,16 @@
 // engine_tt.cpp -- level driver with the transposition table.
 //
 // Downsweep: every expand-and-write level probes its children's buckets and
-// stores only the misses, each with both keys and its parent's index; the
-// last interior level is expanded in leaf mode into per-node accumulators.
-// Upsweep (on the way back up, level by level and chunk by chunk): every
-// finished node adds its exact count to its parent and is stored in the
-// table at its remaining depth.  Hits never change a count: the stored value
-// is the node's exact perft, verified by a second 64-bit key.  This replaces
-// the reference's per-depth table set (launcher.h:62-126, 155-320) and the
-// hash kernels of perft_bb.h:664-961.
+// stores only the misses, each with both keys and its parent's index -- the
+// first child of each position only: later children of the same position in
+// the level are listed as duplicates; the last interior level is expanded in
+// leaf mode into per-node accumulators.  Upsweep (on the way back up, level
+// by level and chunk by chunk): every finished node adds its exact count to
+// its parent and is stored in the table at its remaining depth, and every
+// duplicate adds its original's count to its own parent.  Hits never change
+// a count: the stored value is the node's exact perft, verified by a second
+// 64-bit key.  This replaces the reference's per-depth table set
+// (launcher.h:62-126, 155-320) and the hash kernels of perft_bb.h:664-961.
 #include <string.h>
 
 #include <algorithm>
@@ -57,11 +59,15 @@ bool Engine::alloc_tlevel(u64 n, TLevel* lv) {
     lv->parent = reinterpret_cast<u32*>(alloc(n * 4));
     lv->acc = reinterpret_cast<unsigned long long*>(alloc(n * 8));
     lv->bsum = nullptr;
+    lv->dup = nullptr;
+    lv->dup_n = 0;
+    lv->dpar = lv->dslot = nullptr;
+    lv->ndup = 0;
     return lv->b && lv->root && lv->k1 && lv->k2 && lv->parent && lv->acc;
 }
 
-pf_status Engine::expand_tt(int mode, const TLevel& lv, u64 off, u64 m, const TLevel* nx, u64 cap, int L,
-                            u64* written, u64* next_total, DupEntry* dup, u64 dup_n) {
+pf_status Engine::expand_tt(int mode, const TLevel& lv, u64 off, u64 m, TLevel* nx, u64 cap, int L, u64* written,
+                            u64* next_total) {
     if (!m) {
         if (written) *written = 0;
         if (next_total) *next_total = 0;
@@ -96,8 +102,13 @@ pf_status Engine::expand_tt(int mode, const TLevel& lv, u64 off, u64 m, const TL
     a.tile_ctr = d_ctrl + 1;
     a.err = d_ctrl + 0;
     a.stats = d_ttstats;
-    a.dup = dup;
-    a.dup_n = dup_n;
+    if (mode == kWrite && nx->dup) {
+        a.dup = nx->dup;
+        a.dup_n = nx->dup_n;
+        a.dup_count = d_ctrl + 3;
+        a.dup_par = nx->dpar;
+        a.dup_slot = nx->dslot;
+    }
     if (timing) PF_HIP(hipEventRecord(ev0, stream));
     PF_HIP(launch_expand_tt(backend, d_tab, d_z, a, tt_grid[backend], stream));
     if (timing) PF_HIP(hipEventRecord(ev1, stream));
@@ -109,7 +120,11 @@ pf_status Engine::expand_tt(int mode, const TLevel& lv, u64 off, u64 m, const TL
     const u32* c = reinterpret_cast<const u32*>(h_pinned);
     if (c[0]) return PF_ENOMEM;
     if (written) *written = c[2];
-    if (mode == kWrite) st.frontier_nodes += c[2];
+    if (mode == kWrite) {
+        st.frontier_nodes += c[2];
+        st.dup_nodes += c[3];
+        nx->ndup = c[3];
+    }
     if (next_total) *next_total = h_pinned[4];
     if (timing) {
         float ms = 0;
@@ -118,23 +133,6 @@ pf_status Engine::expand_tt(int mode, const TLevel& lv, u64 off, u64 m, const TL
         run_expand_ms += ms;
         run_expand_launches++;
     }
-    return PF_OK;
-}
-
-// Duplicates among the n nodes of a freshly written level whose children
-// claimed slots of `dup` (k_dedup); *removed_moves = the moves they no
-// longer contribute to the next level.
-pf_status Engine::dedup_level(const TLevel& lv, u64 n, DupEntry* dup, u64 dup_n, u64* removed_moves) {
-    *removed_moves = 0;
-    if (!n) return PF_OK;
-    PF_HIP(hipMemsetAsync(d_ttstats + 4, 0, 2 * sizeof(unsigned long long), stream));
-    PF_HIP(launch_dedup(lv.b, lv.k1, lv.k2, lv.acc, (u32)n, dup, dup_n, d_ttstats + 4, stream));
-    st.kernel_launches++;
-    PF_HIP(hipMemcpyAsync(h_pinned + 8, d_ttstats + 4, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                          stream));
-    PF_HIP(hipStreamSynchronize(stream));
-    *removed_moves = h_pinned[8];
-    st.dup_nodes += h_pinned[9];
     return PF_OK;
 }
 
@@ -148,32 +146,41 @@ static u64 dup_slots(bool on, u64 n, int ply) {
 pf_status Engine::descend_tt(const TLevel& lv, u64 off, u64 m, u64 n_next, int L) {
     if (!m) return PF_OK;
     if (depth - L == 2) return expand_tt(kLeaf, lv, off, m, nullptr, 0, L, nullptr, nullptr);
+    // the children's level, and for ply >= 3 their duplicate table and list,
+    // live until the level's upsweep
     const u64 dn = dup_slots(dedup, n_next, L + 1);
-    const u64 need = n_next * kTNodeBytes + dn * sizeof(DupEntry) + 9 * kAlignTT;
+    const u64 need = n_next * kTNodeBytes + (dn ? dn * sizeof(DupEntry) + n_next * 8 : 0) + 10 * kAlignTT;
     const u64 free_b = arena_bytes - arena_top;
     if (need <= free_b && n_next <= 0xFFFFFFFFull) {
         char* mark = arena + arena_top;
         TLevel nx;
         if (!alloc_tlevel(n_next, &nx)) { release(mark); return PF_ENOMEM; }
-        // the duplicate table lives above the new level until k_dedup has run
-        char* dmark = arena + arena_top;
-        DupEntry* dup = dn ? reinterpret_cast<DupEntry*>(alloc(dn * sizeof(DupEntry))) : nullptr;
-        if (dn && !dup) { release(mark); return PF_ENOMEM; }
-        pf_status s = PF_OK;
-        if (dup && hipMemsetAsync(dup, 0, dn * sizeof(DupEntry), stream) != hipSuccess) s = PF_EDEVICE;
-        u64 written = 0, n_next2 = 0;
-        if (s == PF_OK) s = expand_tt(kWrite, lv, off, m, &nx, n_next, L, &written, &n_next2, dup, dn);
-        if (s == PF_OK && dup) {
-            u64 removed = 0;
-            s = dedup_level(nx, written, dup, dn, &removed);
-            n_next2 -= removed;
+        if (dn) {
+            nx.dup = reinterpret_cast<DupEntry*>(alloc(dn * sizeof(DupEntry)));
+            nx.dup_n = dn;
+            nx.dpar = reinterpret_cast<u32*>(alloc(n_next * 4));
+            nx.dslot = reinterpret_cast<u32*>(alloc(n_next * 4));
+            if (!nx.dup || !nx.dpar || !nx.dslot) { release(mark); return PF_ENOMEM; }
         }
-        release(dmark);
+        pf_status s = PF_OK;
+        if (nx.dup && hipMemsetAsync(nx.dup, 0, dn * sizeof(DupEntry), stream) != hipSuccess) s = PF_EDEVICE;
+        u64 written = 0, n_next2 = 0;
+        if (s == PF_OK) s = expand_tt(kWrite, lv, off, m, &nx, n_next, L, &written, &n_next2);
         if (s == PF_OK) s = descend_tt(nx, 0, written, n_next2, L + 1);
-        if (s == PF_OK && written) {
-            if (launch_upsweep(nx.parent, nx.k1, nx.k2, nx.acc, (u32)written, lv.acc, d_tt, tt_buckets - 1,
-                               depth - (L + 1), d_ttstats + 2, stream) != hipSuccess)
-                s = PF_EDEVICE;
+        if (s == PF_OK && written + nx.ndup) {
+            UpSeg u = {};
+            u.parent = nx.parent;
+            u.k1 = nx.k1;
+            u.k2 = nx.k2;
+            u.acc = nx.acc;
+            u.parent_acc = lv.acc;
+            u.dup = nx.dup;
+            u.dup_par = nx.dpar;
+            u.dup_slot = nx.dslot;
+            u.n = (u32)written;
+            u.ndup = (u32)nx.ndup;
+            u.depth = depth - (L + 1);
+            if (launch_upsweep(u, d_tt, tt_buckets - 1, d_ttstats + 2, stream) != hipSuccess) s = PF_EDEVICE;
             st.kernel_launches++;
         }
         release(mark);
@@ -232,9 +239,13 @@ pf_status Engine::perft_levels_tt(const pf_pos48* pos, int n, int d, u64* out) {
     s = descend_tt(r, 0, (u64)n, n1, 0);
     if (s != PF_OK) return fail(s);
     // the roots themselves, at full depth (answers repeated requests at once)
-    if (launch_upsweep(nullptr, r.k1, r.k2, r.acc, (u32)n, nullptr, d_tt, tt_buckets - 1, d, d_ttstats + 2, stream) !=
-        hipSuccess)
-        return fail(PF_EDEVICE);
+    UpSeg ru = {};
+    ru.k1 = r.k1;
+    ru.k2 = r.k2;
+    ru.acc = r.acc;
+    ru.n = (u32)n;
+    ru.depth = d;
+    if (launch_upsweep(ru, d_tt, tt_buckets - 1, d_ttstats + 2, stream) != hipSuccess) return fail(PF_EDEVICE);
     unsigned long long ts[4];
     if (hipMemcpyAsync(out, r.acc, (size_t)n * 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
         hipMemcpyAsync(ts, d_ttstats, sizeof(ts), hipMemcpyDeviceToHost, stream) != hipSuccess ||

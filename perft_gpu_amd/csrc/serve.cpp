@@ -14,13 +14,13 @@
 // stack, so that it can advance one GPU action at a time.
 //
 // One scheduler step = one replay of a captured graph:
-//   H2D step table -> roots kernel (new requests: table probe at full depth,
-//   move count) -> fill kernel (empties the duplicate tables of this step's
+//   H2D step table -> start kernel (new requests: table probe at full depth,
+//   move count; empties the duplicate tables and block sums of this step's
 //   write segments) -> step kernel (the expand/leaf levels of every active
-//   job in one persistent launch over a shared tile queue) -> duplicate
-//   kernel (in-level duplicates of every written level, as in the level
-//   driver) -> upsweep kernel (the finished levels of every job) -> D2H of
-//   per-segment outcomes and of the slot accumulators.
+//   job in one persistent launch over a shared tile queue; duplicates are
+//   listed as they are made) -> kUpRounds upsweep kernels (finished levels
+//   of every job, each round the levels just above the previous round's) ->
+//   D2H of per-segment outcomes and of the slot accumulators.
 // Requests that arrive while a step runs join the next step; their work
 // overlaps on the device with everyone else's.  All jobs share the lockless
 // transposition table.
@@ -126,8 +126,8 @@ void Server::stop() {
 
 // Predicted peak frontier bytes of a request: the written levels are plies
 // 1..depth-2 (the last is expanded in leaf mode), each with 80 B per node plus
-// a 24-B duplicate-table share, grown by the root's move count per ply (held
-// in [16, 48]) and a quarter of headroom.
+// 32 B of duplicate table and list and a block-sum share, grown by the root's
+// move count per ply (held in [16, 48]) and a quarter of headroom.
 u64 Server::predict_bytes(const pf_pos48& pos, int depth) const {
     if (job_bytes_) return job_bytes_;
     const u64 c = count_moves(to_board(pos), KSBackend{});
@@ -139,7 +139,7 @@ u64 Server::predict_bytes(const pf_pos48& pos, int depth) const {
         sum += lvl;
         lvl *= bf;
     }
-    double bytes = sum * (double)(kTNode + 24) * 1.25 + 16.0 * kAl;
+    double bytes = sum * (double)(kTNode + 33) * 1.25 + 16.0 * kAl;
     const double lo = (double)(1u << 20), hi = (double)max_region_;
     bytes = std::min(hi, std::max(lo, bytes));
     return ((u64)bytes + kAl - 1) & ~(kAl - 1);
@@ -289,13 +289,17 @@ int Server::chain_up(Job& j, int k) {
     for (; k >= 0 && n < kUpRounds; k--) {
         if (nup_planned_ >= (u32)kMaxUps) break;
         Frame& p = j.st[(size_t)k];
-        UpSeg u;
+        UpSeg u = {};
         u.parent = p.nx.parent;
         u.k1 = p.nx.k1;
         u.k2 = p.nx.k2;
         u.acc = p.nx.acc;
         u.parent_acc = p.lv.acc;
+        u.dup = p.nx.dup;
+        u.dup_par = p.nx.dpar;
+        u.dup_slot = p.nx.dslot;
         u.n = (u32)p.written;
+        u.ndup = (u32)p.nx.ndup;
         u.depth = j.depth - p.L - 1;
         up_[(size_t)n].push_back(u);
         nup_planned_++;
@@ -370,20 +374,24 @@ bool Server::plan(Job& j) {
                 f.nx = Engine::TLevel{};
                 j.chain = chain_up(j, (int)j.st.size() - 2);
             } else {
-                // children at ply L+1 get a duplicate table above their level
-                // for the length of this step
+                // children at ply L+1 get a duplicate table and list (24 + 8 B
+                // per child), which live with their level
                 const bool dd = e_.dedup && e_.d_tt && f.L + 1 >= kDupMinPly;  // keys exist only with a table
-                const u64 node_b = kTNode + (dd ? 24 : 0);
+                const u64 node_b = kTNode + (dd ? 32 : 0);
                 const u64 free_b = j.cap - j.top;
                 u64 cap;
-                if (!f.chunking && f.n_next * node_b + 10 * kAl <= free_b / 2 + free_b / 4) {
+                // a level that the leaf pass will finish needs nothing below it:
+                // it may take all the free memory; others leave a quarter
+                const u64 room = (r - 1 == 2) ? free_b : free_b / 2 + free_b / 4;
+                if (!f.chunking && f.n_next * node_b + 10 * kAl <= room) {
                     cap = f.n_next;
                 } else {
                     if (!f.chunking) {
                         f.chunking = true;
                         f.end = f.off + f.m;
                     }
-                    if (!chunk(j, f, (free_b / 2) / node_b, &cap)) return false;
+                    const u64 budget = (r - 1 == 2) ? free_b - std::min<u64>(free_b, 16 * kAl) : free_b / 2;
+                    if (!chunk(j, f, budget / node_b, &cap)) return false;
                     a.n_in = (u32)f.m;
                 }
                 f.mark = j.top;
@@ -396,13 +404,23 @@ bool Server::plan(Job& j) {
                     j.top = f.mark;
                     return false;
                 }
-                f.level_top = j.top;
                 if (dd && cap >= 2) {
                     const u64 dn = dup_table_slots(cap);
-                    DupEntry* dup = reinterpret_cast<DupEntry*>(take(j, dn * sizeof(DupEntry)));
-                    if (dup) {  // without room the level simply keeps its duplicates
-                        a.dup = dup;
+                    const u64 top0 = j.top;
+                    f.nx.dup = reinterpret_cast<DupEntry*>(take(j, dn * sizeof(DupEntry)));
+                    f.nx.dpar = reinterpret_cast<u32*>(take(j, cap * 4));
+                    f.nx.dslot = reinterpret_cast<u32*>(take(j, cap * 4));
+                    if (f.nx.dup && f.nx.dpar && f.nx.dslot) {
+                        f.nx.dup_n = dn;
+                        a.dup = f.nx.dup;
                         a.dup_n = dn;
+                        a.dup_count = &d_tab_->out[si].ndup;
+                        a.dup_par = f.nx.dpar;
+                        a.dup_slot = f.nx.dslot;
+                    } else {  // without room the level simply keeps its duplicates
+                        j.top = top0;
+                        f.nx.dup = nullptr;
+                        f.nx.dpar = f.nx.dslot = nullptr;
                     }
                 }
                 f.cap = cap;
@@ -560,7 +578,7 @@ void Server::run() {
             for (const UpSeg& u : up_[(size_t)r]) {
                 t.up[t.nup] = u;
                 t.up_prefix[t.nup] = acc_u;
-                acc_u += u.n;
+                acc_u += u.n + u.ndup;  // nodes, then duplicates
                 t.nup++;
             }
         }
@@ -603,15 +621,15 @@ void Server::run() {
                 finish(j, 0, PF_ENOMEM);
                 continue;
             }
-            step_dups += o.dups;
+            step_dups += o.ndup;
             Frame& f = j.st.back();
             if (f.state == ST_LEAF) {
                 j.st.pop_back();
                 if (j.st.empty() || after_chain(j, j.chain)) finish(j, h_acc_[j.slot], PF_OK);
                 continue;
             }
-            j.top = f.level_top;  // the duplicate table has served
-            if (o.out_count == 0) {
+            f.nx.ndup = o.ndup;
+            if (o.out_count == 0) {  // every child was answered by the table
                 f.written = 0;
                 f.state = ST_UPDONE;
             } else {  // descend into the written children
@@ -621,7 +639,7 @@ void Server::run() {
                 c.lv = f.nx;
                 c.off = 0;
                 c.m = o.out_count;
-                c.n_next = o.total - o.removed;
+                c.n_next = o.total;
                 c.L = f.L + 1;
                 c.state = ST_START;
                 j.st.push_back(c);

@@ -32,7 +32,6 @@ class Server {
         bool chunking;
         Engine::TLevel nx;
         u64 cap, mark, written;
-        u64 level_top;  // stack top just above the children's level
         std::vector<u32> hbsum;  // host copy of lv.bsum, fetched when the level first chunks
     };
     struct Job {

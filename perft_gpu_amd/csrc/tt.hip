@@ -4,14 +4,17 @@
 // LDS list, one (parent, move) pair per lane), plus:
 //   * every child carries two incremental Zobrist keys (tt.h) and the index
 //     of its parent;
-//   * expand-and-write: each child (remaining depth >= 2) probes its bucket;
-//     a hit adds the stored count to the parent's accumulator and the child is
-//     not materialised (replaces makemove_and_count_moves_single_level_hash128b,
-//     perft_bb.h:664-786); misses are compacted with a wave ballot and written
+//   * expand-and-write: each child of remaining depth >= kMinTTDepth probes
+//     its bucket; a hit adds the stored count to the parent's accumulator and
+//     the child is not materialised (replaces
+//     makemove_and_count_moves_single_level_hash128b, perft_bb.h:664-786);
+//     a miss claims its slot in the level's duplicate table: the first child
+//     of a position is compacted with a wave ballot and written, later ones
+//     only join the level's duplicate list
 //   * leaf: each parent's accumulator receives perft(parent, 2)
-//   * upsweep: a finished level adds its counts into the parents' accumulators
-//     and stores them in the table (calcPerftNFromPerftNminus1_hash128b,
-//     perft_bb.h:854-961)
+//   * upsweep: a finished level adds its counts, and its duplicates their
+//     claimants' counts, into the parents' accumulators and stores the nodes
+//     in the table (calcPerftNFromPerftNminus1_hash128b, perft_bb.h:854-961)
 #include <hip/hip_runtime.h>
 
 #include "tt_kernels.h"
@@ -64,25 +67,26 @@ __device__ __forceinline__ void tt_store(uint4* tt, u64 mask, u64 k1, u64 k2, in
 }
 
 // Slot of a key in a duplicate table of n slots (n < 2^32): multiply-shift
-// range reduction of the low half of the first key, so that the high half
-// stays free to verify a match.
-__device__ __forceinline__ u64 dup_slot(u64 k1, u64 n) { return ((k1 & 0xFFFFFFFFull) * n) >> 32; }
+// range reduction of the high half of the first key.
+__device__ __forceinline__ u64 dup_slot(u64 k1, u64 n) { return ((k1 >> 32) * n) >> 32; }
 
-// A written child claims the first empty slot of its probe sequence, unless a
-// node with its second key already holds one.  The claimant publishes its
-// index; other nodes read it only in a later launch (k_dedup).
-__device__ __forceinline__ void dup_insert(DupEntry* t, u64 n, u64 k1, u64 k2, u32 idx) {
-    if (!k2) return;
+// A made child claims the first empty slot of its probe sequence.  Returns 0
+// when it claimed *slot (the first child of its position in this level: it
+// is written and publishes its index there), 1 when an earlier child with
+// its keys holds *slot (a duplicate), 2 when the probe sequence is full or
+// the key is 0 (written, no duplicate detection).
+__device__ __forceinline__ int dup_claim(DupEntry* t, u64 n, u64 k1, u64 k2, u64* slot) {
+    if (!k2) return 2;
     u64 s = dup_slot(k1, n);
     for (int p = 0; p < kDupProbes; p++) {
         const unsigned long long old = atomicCAS(&t[s].key, 0ull, (unsigned long long)k2);
-        if (old == 0) {
-            t[s].val = (k1 & 0xFFFFFFFF00000000ull) | idx;
-            return;
+        if (old == 0 || old == k2) {
+            *slot = s;
+            return old == 0 ? 0 : 1;
         }
-        if (old == k2) return;
         s = (s + 1 == n) ? 0 : s + 1;
     }
+    return 2;
 }
 
 struct TTLds {
@@ -168,8 +172,9 @@ __device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTa
                 }
                 Board c = {};
                 u32 cc = 0;
-                bool keep = false;
-                u64 ck1 = 0, ck2 = 0;
+                bool keep = false, probed = false, hit = false, isdup = false;
+                int claim = 2;
+                u64 ck1 = 0, ck2 = 0, dslot = 0;
                 if (valid) {
                     const Board pb = s.b[par];
                     c = make_move(pb, mvv);
@@ -180,10 +185,14 @@ __device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTa
                             ck1 = zobrist_child(zt->k[0], s.k1[par], pb, mvv, black, c);
                             ck2 = zobrist_child(zt->k[1], s.k2[par], pb, mvv, black, c);
                         }
-                        if (a.tt && tt_probe(a.tt, a.tt_mask, ck1, ck2, a.child_depth, &v)) {
+                        probed = a.tt && a.child_depth >= kMinTTDepth;
+                        hit = probed && tt_probe(a.tt, a.tt_mask, ck1, ck2, a.child_depth, &v);
+                        if (hit) {
                             atomicAdd(&s.hit[par], (unsigned long long)v);
                         } else {
-                            keep = true;
+                            if (a.dup) claim = dup_claim(a.dup, a.dup_n, ck1, ck2, &dslot);
+                            isdup = claim == 1;
+                            keep = !isdup;
                         }
                     }
                     if (a.mode == kLeaf || keep) {
@@ -193,12 +202,12 @@ __device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTa
                 }
                 if (a.mode == kWrite) {
                     const u64 bal = __ballot(keep);
-                    const u64 vbal = __ballot(valid);
                     const u32 nk = (u32)__popcll(bal);
-                    if (lane == 0) {
-                        const u32 nv = (u32)__popcll(vbal);
-                        atomicAdd(&s.nprobe, nv);
-                        atomicAdd(&s.nhit, nv - nk);
+                    const u32 np_ = (u32)__popcll(__ballot(probed));
+                    const u32 nh = (u32)__popcll(__ballot(hit));
+                    if (lane == 0 && np_) {
+                        atomicAdd(&s.nprobe, np_);
+                        atomicAdd(&s.nhit, nh);
                     }
                     u32 base = 0;
                     if (lane == 0 && nk) base = atomicAdd(a.out_count, nk);
@@ -215,9 +224,27 @@ __device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTa
                             }
                             a.out_parent[pos] = a.in_base + p0 + par;
                             a.out_acc[pos] = 0;
-                            if (a.dup) dup_insert(a.dup, a.dup_n, ck1, ck2, (u32)pos);
+                            if (claim == 0) a.dup[dslot].val = pos;  // publish the claimant
                         } else {
                             atomicOr(a.err, 1u);
+                        }
+                    }
+                    if (a.dup) {
+                        const u64 dbal = __ballot(isdup);
+                        const u32 nd = (u32)__popcll(dbal);
+                        if (nd) {
+                            u32 dbase = 0;
+                            if (lane == 0) dbase = atomicAdd(a.dup_count, nd);
+                            dbase = __shfl(dbase, 0, 64);
+                            const u64 dpos = (u64)dbase + (u32)__popcll(dbal & ((1ULL << lane) - 1));
+                            if (isdup) {
+                                if (dpos < a.out_cap) {
+                                    a.dup_par[dpos] = a.in_base + p0 + par;
+                                    a.dup_slot[dpos] = (u32)dslot;
+                                } else {
+                                    atomicOr(a.err, 1u);
+                                }
+                            }
                         }
                     }
                     if (a.out_bsum && nk) {
@@ -292,7 +319,7 @@ __global__ __launch_bounds__(256) void k_roots_tt(const SliderTables* tabs, Boar
     if (i >= n) return;
     const Board b = roots[i];
     u64 v = 0;
-    if (tt && tt_probe(tt, mask, k1[i], k2[i], depth, &v)) {
+    if (tt && depth >= kMinTTDepth && tt_probe(tt, mask, k1[i], k2[i], depth, &v)) {
         acc[i] = v;
         roots[i] = with_count(b, 0);  // answered: nothing to expand
         return;
@@ -303,73 +330,31 @@ __global__ __launch_bounds__(256) void k_roots_tt(const SliderTables* tabs, Boar
     atomicAdd(total, (unsigned long long)c);
 }
 
-// Duplicates within a freshly written level (replaces writeIndexInHashForDuplicates
-// + checkandMarkDuplicates, perft_bb.h:788-846): a node whose keys match the
-// claimant of its slot, and is not the claimant, is a duplicate.  It loses its
-// moves (count byte 0: nothing below it is expanded) and its accumulator
-// records the original, whose exact count it takes in the upsweep
-// (perft_bb.h:871-878).  removed[0] += the duplicates' move counts,
-// removed[1] += the number of duplicates.
-// Node i of a level of n nodes: returns its move count if it is a duplicate
-// (and marks it), else 0 with *dup = false.
-__device__ __forceinline__ u32 dedup_node(Board* b, const u64* k1, const u64* k2, unsigned long long* acc, u32 i,
-                                          u32 n, const DupEntry* t, u64 tn, bool* dup) {
-    *dup = false;
-    const u64 key2 = k2[i];
-    if (!key2) return 0;
-    const u64 key1 = k1[i];
-    u64 s = dup_slot(key1, tn);
-    for (int p = 0; p < kDupProbes; p++) {
-        const DupEntry e = t[s];
-        if (e.key == 0) return 0;
-        if (e.key == key2) {
-            const u32 o = (u32)e.val;
-            if ((e.val >> 32) != (key1 >> 32) || o == i || o >= n) return 0;
-            const u64 pw = b[i].pawns;
-            b[i].pawns = pw & ~(0xFFull << 56);
-            acc[i] = kDupBit | o;
-            *dup = true;
-            return (u32)(pw >> 56);
+// Item k of a finished level's upsweep (calcPerftNFromPerftNminus1_hash128b,
+// perft_bb.h:854-961): k < n is node k, which adds its exact count to its
+// parent and is stored in the table; n <= k < n + ndup is duplicate k - n,
+// which adds the count of the node that claimed its slot.  Returns 1 for a
+// store.
+__device__ __forceinline__ u32 upsweep_item(const UpSeg& u, u32 k, uint4* tt, u64 mask) {
+    if (k < u.n) {
+        const unsigned long long v = u.acc[k];
+        if (u.parent_acc) atomicAdd(&u.parent_acc[u.parent[k]], v);
+        if (tt && u.depth >= kMinTTDepth) {
+            tt_store(tt, mask, u.k1[k], u.k2[k], u.depth, v);
+            return 1;
         }
-        s = (s + 1 == tn) ? 0 : s + 1;
+        return 0;
     }
+    const u32 d = k - u.n;
+    const u32 c = (u32)u.dup[u.dup_slot[d]].val;
+    if (u.parent_acc) atomicAdd(&u.parent_acc[u.dup_par[d]], u.acc[c]);
     return 0;
 }
 
-__global__ __launch_bounds__(256) void k_dedup(Board* b, const u64* k1, const u64* k2, unsigned long long* acc, u32 n,
-                                               const DupEntry* t, u64 tn, unsigned long long* removed) {
-    const u32 i = blockIdx.x * 256 + threadIdx.x;
-    unsigned long long mv = 0, nd = 0;
-    if (i < n) {
-        bool d = false;
-        mv = dedup_node(b, k1, k2, acc, i, n, t, tn, &d);
-        nd = d ? 1 : 0;
-    }
-    mv = t_wave_sum64(mv);
-    nd = t_wave_sum64(nd);
-    if ((threadIdx.x & 63) == 0 && nd) {
-        atomicAdd(&removed[0], mv);
-        atomicAdd(&removed[1], nd);
-    }
-}
-
-// Finished level: add each node's count to its parent and store it.  A
-// duplicate takes its original's count and stores nothing.
-__global__ __launch_bounds__(256) void k_upsweep(const u32* parent, const u64* k1, const u64* k2,
-                                                 const unsigned long long* acc, u32 n, unsigned long long* parent_acc,
-                                                 uint4* tt, u64 mask, int depth, unsigned long long* stores) {
+__global__ __launch_bounds__(256) void k_upsweep(UpSeg u, uint4* tt, u64 mask, unsigned long long* stores) {
     const u32 i = blockIdx.x * 256 + threadIdx.x;
     unsigned long long st = 0;
-    if (i < n) {
-        unsigned long long v = acc[i];
-        const bool dup = (v & kDupBit) != 0;
-        if (dup) v = acc[(u32)v];
-        if (parent_acc) atomicAdd(&parent_acc[parent[i]], v);
-        if (tt && !dup) {
-            tt_store(tt, mask, k1[i], k2[i], depth, v);
-            st = 1;
-        }
-    }
+    if (i < u.n + u.ndup) st = upsweep_item(u, i, tt, mask);
     if (stores) {
         st = t_wave_sum64(st);
         if ((threadIdx.x & 63) == 0 && st) atomicAdd(stores, st);
@@ -415,45 +400,9 @@ __global__ __launch_bounds__(kTWG) void k_step(const SliderTables* tabs, const Z
     }
 }
 
-// Duplicates among the children that this step's write segments wrote
-// (k_dedup per segment); the segment's next-level total loses their moves.
-__device__ void dedup_step_segments(StepTable* t) {
-    const u32 stride = gridDim.x * 256;
-    for (u32 si = 0; si < t->nseg; si++) {
-        const TTArgs& a = t->seg[si];
-        if (!a.dup) continue;
-        // children past the capacity were not written (the segment reports an error)
-        const u32 n = (u32)min((u64)t->out[si].out_count, a.out_cap);
-        unsigned long long mv = 0, nd = 0;
-        // every lane runs the same trip count, so the wave sums below are uniform
-        for (u32 base = blockIdx.x * 256; base < n; base += stride) {
-            const u32 i = base + threadIdx.x;
-            u32 m = 0;
-            if (i < n) {
-                bool d = false;
-                m = dedup_node(a.out, a.out_k1, a.out_k2, a.out_acc, i, n, a.dup, a.dup_n, &d);
-                nd += d ? 1 : 0;
-            }
-            mv += m;
-            if (a.out_bsum) {  // a wave's 64 nodes lie in one block
-                const unsigned long long wm = t_wave_sum64(m);
-                if ((threadIdx.x & 63) == 0 && wm) atomicSub(&a.out_bsum[i >> kBsumShift], (u32)wm);
-            }
-        }
-        mv = t_wave_sum64(mv);
-        nd = t_wave_sum64(nd);
-        if ((threadIdx.x & 63) == 0 && nd) {
-            atomicAdd(&t->out[si].removed, mv);
-            atomicAdd(&t->out[si].dups, nd);
-        }
-    }
-}
-
-// Upsweep round `round`: thread i of the round's concatenated segments finds
-// its segment by search.  Round 0 first runs the duplicate pass over the
-// levels written in this step (no upsweep of this step reads them).
+// Upsweep round `round`: thread i of the round's concatenated segments (nodes
+// and duplicates) finds its segment by search.
 __global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 mask, int round) {
-    if (round == 0) dedup_step_segments(t);
     const u32 s0 = t->up_round[round], s1 = t->up_round[round + 1];
     const u32 first = t->up_prefix[s0], last = t->up_prefix[s1];
     for (u32 i = first + blockIdx.x * 256 + threadIdx.x; i < last; i += gridDim.x * 256) {
@@ -462,13 +411,7 @@ __global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 ma
             const u32 mid = (lo + hi) >> 1;
             if (t->up_prefix[mid] <= i) lo = mid; else hi = mid;
         }
-        const UpSeg& u = t->up[lo];
-        const u32 k = i - t->up_prefix[lo];
-        unsigned long long v = u.acc[k];
-        const bool dup = (v & kDupBit) != 0;
-        if (dup) v = u.acc[(u32)v];
-        if (u.parent_acc) atomicAdd(&u.parent_acc[u.parent[k]], v);
-        if (tt && !dup) tt_store(tt, mask, u.k1[k], u.k2[k], u.depth, v);
+        upsweep_item(t->up[lo], i - t->up_prefix[lo], tt, mask);
     }
 }
 
@@ -488,7 +431,7 @@ __global__ __launch_bounds__(256) void k_pre_step(const SliderTables* tabs, Step
             *r.dst_k2 = r.k2;
         }
         u64 v = 0;
-        if (tt && r.depth >= 2 && tt_probe(tt, mask, r.k1, r.k2, r.depth, &v)) {
+        if (tt && r.depth >= kMinTTDepth && tt_probe(tt, mask, r.k1, r.k2, r.depth, &v)) {
             *r.acc = v;
             *r.dst_b = with_count(b, 0);  // answered: nothing below it is expanded
             t->rout[gi].hit = 1;
@@ -578,19 +521,11 @@ hipError_t launch_roots_tt(int backend, const SliderTables* tabs, Board* roots, 
     return hipGetLastError();
 }
 
-hipError_t launch_dedup(Board* b, const u64* k1, const u64* k2, unsigned long long* acc, u32 n, const DupEntry* t,
-                        u64 tn, unsigned long long* removed, hipStream_t s) {
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_dedup, dim3((n + 255) / 256), dim3(256), 0, s, b, k1, k2, acc, n, t, tn, removed);
-    return hipGetLastError();
-}
 
-hipError_t launch_upsweep(const u32* parent, const u64* k1, const u64* k2, const unsigned long long* acc, u32 n,
-                          unsigned long long* parent_acc, uint4* tt, u64 mask, int depth, unsigned long long* stores,
-                          hipStream_t s) {
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_upsweep, dim3((n + 255) / 256), dim3(256), 0, s, parent, k1, k2, acc, n, parent_acc, tt, mask,
-                       depth, stores);
+hipError_t launch_upsweep(const UpSeg& u, uint4* tt, u64 mask, unsigned long long* stores, hipStream_t s) {
+    const u32 items = u.n + u.ndup;
+    if (!items) return hipSuccess;
+    hipLaunchKernelGGL(k_upsweep, dim3((items + 255) / 256), dim3(256), 0, s, u, tt, mask, stores);
     return hipGetLastError();
 }
 

@@ -11,15 +11,25 @@ namespace pf {
 template <int B>
 using SlOfT = SlOf<B>;
 
-// In-level duplicate detection (open addressing, linear probing): `key` is
-// the node's second Zobrist key (0 = empty slot, claimed by compare-and-swap),
-// `val` = high half of the first key << 32 | the claimant's index in its level.
+// Table probes and stores only for nodes with at least this remaining depth:
+// a node of remaining depth 2 costs one count pass over its children, about
+// what a random 32-byte probe plus its share of the store costs.
+constexpr int kMinTTDepth = 3;
+
+// In-level duplicate detection, at the moment a child is made (open
+// addressing, linear probing).  `key` is the child's second Zobrist key
+// (0 = empty slot), claimed by compare-and-swap; the first child of a
+// position to claim its slot is written to the level and publishes its
+// index in `val`; a later child with the same key (its slot comes from the
+// first key, so a match verifies both) is not written -- it joins the
+// level's duplicate list as (parent, slot) and takes the claimant's count in
+// the upsweep (the reference's writeIndexInHashForDuplicates /
+// checkandMarkDuplicates, perft_bb.h:788-846, and perft_bb.h:871-878).
 struct alignas(16) DupEntry {
     unsigned long long key;
-    unsigned long long val;
+    unsigned long long val;  // index of the claimant in its level
 };
 constexpr int kDupProbes = 32;
-constexpr unsigned long long kDupBit = 1ULL << 63;  // acc of a duplicate: kDupBit | original's index
 constexpr int kDupMinPly = 3;  // no two move sequences shorter than 3 plies reach the same position
 constexpr u32 kBsumShift = 10;  // block sums of move counts over 1024 consecutive nodes
 constexpr u32 kBsumBlock = 1u << kBsumShift;
@@ -57,8 +67,11 @@ struct TTArgs {
     u32* tile_ctr;
     u32* err;
     unsigned long long* stats;   // [0] probes, [1] hits
-    DupEntry* dup;               // written children claim a slot here (null: no duplicate detection)
+    DupEntry* dup;               // children claim a slot here (null: no duplicate detection)
     u64 dup_n;                   // slots (< 2^32)
+    u32* dup_count;              // duplicates listed so far
+    u32* dup_par;                // duplicate list: parent index (in its level) ...
+    u32* dup_slot;               // ... and the slot its claimant holds; capacity out_cap
     u32* out_bsum;               // per kBsumBlock children: sum of their move counts (null: not kept)
     u32 bsum_n;                  // blocks (cleared by the serving step's start kernel)
 };
@@ -76,7 +89,10 @@ struct UpSeg {
     const u64* k2;
     const unsigned long long* acc;
     unsigned long long* parent_acc;
-    u32 n;
+    const DupEntry* dup;         // the level's duplicate list and table
+    const u32* dup_par;
+    const u32* dup_slot;
+    u32 n, ndup;                 // nodes, then duplicates: n + ndup items
     int depth;
 };
 struct RootSeg {  // payload travels in the table; the kernel scatters it
@@ -94,8 +110,8 @@ struct SegOut {
     u32 out_count;
     u32 err;
     unsigned long long total;
-    unsigned long long removed;  // moves of the children found to be duplicates
-    unsigned long long dups;     // children found to be duplicates
+    u32 ndup;  // children found to duplicate a written sibling of their level
+    u32 pad;
 };
 struct RootOut {
     u32 hit;
@@ -109,7 +125,7 @@ constexpr int kUpRounds = 4;
 struct StepTable {
     u32 nseg, nup, nroot, tile_ctr;
     u32 tile_prefix[kMaxSegs + 1];
-    u32 up_prefix[kMaxUps + 1];    // node prefix over all upsweep segments
+    u32 up_prefix[kMaxUps + 1];    // item prefix over all upsweep segments
     u32 up_round[kUpRounds + 1];   // segments [up_round[r], up_round[r+1]) belong to round r
     TTArgs seg[kMaxSegs];
     UpSeg up[kMaxUps];
@@ -127,10 +143,8 @@ hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables
 hipError_t launch_roots_tt(int backend, const SliderTables* tabs, Board* roots, const u64* k1, const u64* k2, u32 n,
                            int depth, const uint4* tt, u64 mask, unsigned long long* acc, unsigned long long* total,
                            hipStream_t s);
-hipError_t launch_dedup(Board* b, const u64* k1, const u64* k2, unsigned long long* acc, u32 n, const DupEntry* t,
-                        u64 tn, unsigned long long* removed, hipStream_t s);
-hipError_t launch_upsweep(const u32* parent, const u64* k1, const u64* k2, const unsigned long long* acc, u32 n,
-                          unsigned long long* parent_acc, uint4* tt, u64 mask, int depth, unsigned long long* stores,
-                          hipStream_t s);
+// A finished level: its n nodes and ndup duplicates add their counts to their
+// parents (parent_acc null: the roots); nodes are stored in the table.
+hipError_t launch_upsweep(const UpSeg& u, uint4* tt, u64 mask, unsigned long long* stores, hipStream_t s);
 
 }  // namespace pf
