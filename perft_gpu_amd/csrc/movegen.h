@@ -119,7 +119,14 @@ PF_HD int slider_count(const SL& sl, u64 B, u64 R, u64 occ, u64 tgt, u64 pinned,
     }
 }
 
-// Number of legal moves (countMoves, MoveGeneratorBitboard.h:2828-3295).
+// Number of legal moves (countMoves + countMovesOutOfCheck,
+// MoveGeneratorBitboard.h:2828-3295).  One branch-free path serves both: out
+// of check `safe` is every square; in single check it is the checker and
+// the squares between it and the king, pinned pieces do not move and a
+// double push counts only where the single push does not resolve the check;
+// in double check `safe` is empty and only the king moves.  (A wave holds 64
+// different positions, so a separate in-check path would run for nearly
+// every wave.)
 template <class SL>
 PF_HD u32 count_moves(const Board& b, const SL& sl) {
     PosInfo pi;
@@ -127,65 +134,71 @@ PF_HD u32 count_moves(const Board& b, const SL& sl) {
     const u64 occ = pi.occ, us = pi.us, them = pi.them, empty = ~occ;
     const u64 P = pi.pawns & us;
     const u64 Pf = P & ~pi.pinned;
-    const int ep = st_ep(b);
+    const int ksq = pi.ksq;
+    const bool check = pi.checkers != 0;
+    const bool dbl = (pi.checkers & (pi.checkers - 1)) != 0;
+    const u64 safe = !check ? ~0ULL : dbl ? 0ULL : (pi.checkers | pi.block);
+    const u64 tgt = ~us & safe;
+    const u64 pinMov = check ? 0ULL : pi.pinned;  // pinned pieces that move along their line
     int n = popc(king_fill(pi.K) & ~us & ~pi.att);
 
-    if (pi.checkers) {
-        // countMovesOutOfCheck (MoveGeneratorBitboard.h:2828-3002)
-        if (pi.checkers & (pi.checkers - 1)) return (u32)n;
-        const u64 safe = pi.checkers | pi.block;
-        const u64 p1 = (Pf << 8) & empty;
-        n += popc(p1 & safe) + 3 * popc(p1 & safe & kRank8);
-        n += popc((((p1 & ~safe) & kRank3) << 8) & empty & safe);
-        const u64 cl = ((Pf << 7) & kNotH) & them & safe, cr = ((Pf << 9) & kNotA) & them & safe;
-        n += popc(cl) + popc(cr) + 3 * (popc(cl & kRank8) + popc(cr & kRank8));
-        if (ep) {
-            const u64 C = bit(32 + ep - 1);
-            if (C == pi.checkers) n += popc((((C << 1) & kNotA) | ((C >> 1) & kNotH)) & Pf);
-        }
-        n += knight_count(b.knights & us & ~pi.pinned, safe);
-        n += slider_count(sl, b.bishops & us & ~pi.pinned, b.rooks & us & ~pi.pinned, occ, safe, 0, pi.ksq);
-        return (u32)n;
-    }
-
-    const int ksq = pi.ksq;
-    const u64 pinF = pi.pinned & file_of(ksq), pinD = pi.pinned & diag_of(ksq), pinA = pi.pinned & anti_of(ksq);
-    // en passant (MoveGeneratorBitboard.h:3058-3107)
-    if (ep) {
-        const u64 T = bit(40 + ep - 1), C = T >> 8;
-        u64 src = (((C << 1) & kNotA) | ((C >> 1) & kNotH)) & P;
-        const u64 eR = b.rooks & them;
-        while (src) {
-            const u64 s = src & (0 - src);
-            src ^= s;
-            if (s & pi.pinned) {
-                const u64 line = (pinD & s) ? diag_of(ksq) : (pinA & s) ? anti_of(ksq) : (pinF & s) ? file_of(ksq) : rank_of(ksq);
-                n += (T & line) ? 1 : 0;
-            } else {
-                const u64 pro = empty | C | s;
-                n += ((att_e(eR, pro) | att_w(eR, pro)) & pi.K) ? 0 : 1;
-            }
-        }
-    }
-    // pawns (MoveGeneratorBitboard.h:3109-3178)
+    // pawns (MoveGeneratorBitboard.h:3109-3178; in check :2892-2940)
     {
         const u64 p1 = (Pf << 8) & empty;
-        n += popc(p1) + 3 * popc(p1 & kRank8);
-        n += popc(((p1 & kRank3) << 8) & empty);
-        const u64 cl = ((Pf << 7) & kNotH) & them, cr = ((Pf << 9) & kNotA) & them;
+        n += popc(p1 & safe) + 3 * popc(p1 & safe & kRank8);
+        const u64 dmask = check ? ~safe : ~0ULL;
+        n += popc((((p1 & dmask) & kRank3) << 8) & empty & safe);
+        const u64 cl = ((Pf << 7) & kNotH) & them & safe, cr = ((Pf << 9) & kNotA) & them & safe;
         n += popc(cl) + popc(cr) + 3 * (popc(cl & kRank8) + popc(cr & kRank8));
+        const u64 pinF = pinMov & file_of(ksq), pinD = pinMov & diag_of(ksq), pinA = pinMov & anti_of(ksq);
         const u64 q1 = ((P & pinF) << 8) & empty;  // file-pinned pawns push along the pin
         n += popc(q1) + popc(((q1 & kRank3) << 8) & empty);
         const u64 qd = (((P & pinD) << 9) & kNotA) & them;  // stays on the a1-h8 pin line
         const u64 qa = (((P & pinA) << 7) & kNotH) & them;  // stays on the a8-h1 pin line
         n += popc(qd) + popc(qa) + 3 * (popc(qd & kRank8) + popc(qa & kRank8));
     }
-    // castling (MoveGeneratorBitboard.h:3180-3198)
+    // en passant (MoveGeneratorBitboard.h:3058-3107; in check :2873-2890)
+    const int ep = st_ep(b);
+    if (ep) {
+        const u64 T = bit(40 + ep - 1), C = T >> 8;
+        const u64 src = (((C << 1) & kNotA) | ((C >> 1) & kNotH)) & P;
+        if (check) {
+            // only when the pawn that just moved is the single checker
+            if (C == pi.checkers) n += popc(src & ~pi.pinned);
+        } else {
+            const u64 eR = b.rooks & them;
+            u64 ok = src & ~pi.pinned;
+            if ((pi.K & kRank5) && (eR & kRank5)) {
+                // both pawns leave the king's rank: the horizontal discovered check test
+                while (ok) {
+                    const u64 s1 = ok & (0 - ok);
+                    ok ^= s1;
+                    const u64 pro = empty | C | s1;
+                    n += ((att_e(eR, pro) | att_w(eR, pro)) & pi.K) ? 0 : 1;
+                }
+            } else {
+                n += popc(ok);
+            }
+            u64 pin = src & pi.pinned;  // a pinned capturer only along its pin line
+            while (pin) {
+                const u64 s1 = pin & (0 - pin);
+                pin ^= s1;
+                const u64 pinF = pi.pinned & file_of(ksq), pinD = pi.pinned & diag_of(ksq);
+                const u64 pinA = pi.pinned & anti_of(ksq);
+                const u64 line = (pinD & s1) ? diag_of(ksq) : (pinA & s1) ? anti_of(ksq)
+                                 : (pinF & s1) ? file_of(ksq) : rank_of(ksq);
+                n += (T & line) ? 1 : 0;
+            }
+        }
+    }
+    // castling (MoveGeneratorBitboard.h:3180-3198), never out of check
     const int r = st_rights(b);
-    n += ((r & 1) && !(occ & kF1G1) && !(pi.att & kF1G1)) ? 1 : 0;
-    n += ((r & 2) && !(occ & kB1D1) && !(pi.att & kC1D1)) ? 1 : 0;
-    n += knight_count(b.knights & us & ~pi.pinned, ~us);
-    n += slider_count(sl, b.bishops & us, b.rooks & us, occ, ~us, pi.pinned, ksq);
+    n += (!check && (r & 1) && !(occ & kF1G1) && !(pi.att & kF1G1)) ? 1 : 0;
+    n += (!check && (r & 2) && !(occ & kB1D1) && !(pi.att & kC1D1)) ? 1 : 0;
+    // knights (pinned ones never move) and sliders (pinned ones along their line, never in check)
+    n += knight_count(b.knights & us & ~pi.pinned, tgt);
+    const u64 frozen = pi.pinned & ~pinMov;
+    n += slider_count(sl, b.bishops & us & ~frozen, b.rooks & us & ~frozen, occ, tgt, pinMov, ksq);
     return (u32)n;
 }
 

@@ -37,10 +37,18 @@ __global__ __launch_bounds__(256) void k_valu(u32 iters, u32 seed, u32* sink) {
                  OPC " %6, %6, %8\n\t" OPC " %7, %7, %8"                                \
                  : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6), "+v"(a7) \
                  : "v"(k))
+// three-source forms: d = op(d, k, k2)
+#define PF_OP8_3(OPC, MOD)                                                                           \
+    asm volatile(OPC " %0, %0, %8, %9" MOD "\n\t" OPC " %1, %1, %8, %9" MOD "\n\t"                   \
+                 OPC " %2, %2, %8, %9" MOD "\n\t" OPC " %3, %3, %8, %9" MOD "\n\t"                   \
+                 OPC " %4, %4, %8, %9" MOD "\n\t" OPC " %5, %5, %8, %9" MOD "\n\t"                   \
+                 OPC " %6, %6, %8, %9" MOD "\n\t" OPC " %7, %7, %8, %9" MOD                          \
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6), "+v"(a7)       \
+                 : "v"(k), "v"(k2))
 template <int OP>
 __global__ __launch_bounds__(256) void k_opc(u32 iters, u32* sink) {
     u32 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, a4 = a0 + 4, a5 = a0 + 5, a6 = a0 + 6, a7 = a0 + 7;
-    const u32 k = blockIdx.x | 3;
+    const u32 k = blockIdx.x | 3, k2 = blockIdx.x * 5u + 1u;
     for (u32 i = 0; i < iters; i++) {
 #pragma unroll
         for (int j = 0; j < 8; j++) {
@@ -50,11 +58,15 @@ __global__ __launch_bounds__(256) void k_opc(u32 iters, u32* sink) {
             if constexpr (OP == 3) PF_OP8("v_bcnt_u32_b32");
             if constexpr (OP == 4) PF_OP8("v_lshlrev_b32");
             if constexpr (OP == 5) PF_OP8("v_mul_lo_u32");
+            if constexpr (OP == 7) PF_OP8_3("v_bitop3_b32", " bitop3:0xf8");
+            if constexpr (OP == 8) PF_OP8_3("v_and_or_b32", "");
+            if constexpr (OP == 9) PF_OP8_3("v_or3_b32", "");
         }
     }
     if ((a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7) == 0x12345678u) sink[0] = 1;
 }
 #undef PF_OP8
+#undef PF_OP8_3
 
 __global__ __launch_bounds__(256) void k_opc64(u32 iters, u32* sink) {
     u64 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3;
@@ -112,7 +124,8 @@ pf_status pf::Engine::probe(int kind, u64 bytes, u64 iters, double* out) {
         *out = (double)grid * 256.0 * it * 64.0 / (ms * 1e-3);
         hipFree(sink);
     } else if (kind == 3) {
-        // iters selects the opcode: 0 add, 1 xor, 2 and, 3 bcnt, 4 lshl32, 5 mul_lo, 6 lshl64
+        // iters selects the opcode: 0 add, 1 xor, 2 and, 3 bcnt, 4 lshl32, 5 mul_lo, 6 lshl64,
+        // 7 bitop3 (d | k & k2), 8 and_or, 9 or3
         u32* sink = nullptr;
         if (hipMalloc(&sink, 4) != hipSuccess) return PF_ENOMEM;
         const u32 it = 2048;
@@ -124,6 +137,9 @@ pf_status pf::Engine::probe(int kind, u64 bytes, u64 iters, double* out) {
         case 3: f = (const void*)k_opc<3>; break;
         case 4: f = (const void*)k_opc<4>; break;
         case 5: f = (const void*)k_opc<5>; break;
+        case 7: f = (const void*)k_opc<7>; break;
+        case 8: f = (const void*)k_opc<8>; break;
+        case 9: f = (const void*)k_opc<9>; break;
         default: f = (const void*)k_opc64; break;
         }
         int per_cu = 0;

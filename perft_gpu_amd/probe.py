@@ -12,7 +12,8 @@ import json
 from . import Engine
 
 KINDS = {"valu": 0, "copy": 1, "random16": 2, "opcodes": 3}
-OPCODES = ["v_add_u32", "v_xor_b32", "v_and_b32", "v_bcnt_u32_b32", "v_lshlrev_b32", "v_mul_lo_u32", "v_lshlrev_b64"]
+OPCODES = ["v_add_u32", "v_xor_b32", "v_and_b32", "v_bcnt_u32_b32", "v_lshlrev_b32", "v_mul_lo_u32", "v_lshlrev_b64",
+           "v_bitop3_b32", "v_and_or_b32", "v_or3_b32"]
 
 
 def main():
