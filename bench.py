@@ -214,6 +214,7 @@ def main():
     ap.add_argument("--c5-positions", type=int, default=4096)
     ap.add_argument("--no-extras", action="store_true", help="skip every comparison run")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU only; --gpus %d is not supported (use --gpus 1)" % args.gpus,
@@ -229,6 +230,17 @@ def main():
     pos = W.positions(cfg)
     eng = pg.Engine(backend=args.backend, arena_bytes=16 << 30)
     log("bench: %d positions depth %d backend %s" % (len(pos), depth, args.backend))
+    tuning = None
+    if not args.no_tune:
+        # the autotuner times every launch shape on this workload and keeps the fastest
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            tf = os.path.join(td, "tune.txt")
+            t = time.perf_counter()
+            eng.tune(pos, depth, tf)
+            tuning = {"seconds": time.perf_counter() - t,
+                      "choice": [l.strip() for l in open(tf) if l.startswith("backend " + args.backend + " ")]}
+        log("tuning: %s" % tuning)
     for i in range(args.warmup):
         counts = eng.perft_batch(pos, depth, mode=pg.MODE_PERSISTENT)
         log("warmup %d done" % i)
@@ -262,7 +274,7 @@ def main():
         "data": "synthetic: seeded random legal playouts from the initial position",
         "config": {"workload": cfg["name"], "positions": len(pos), "depth": depth, "tt": "off",
                    "backend": args.backend, "mode": "persistent", "leaves_per_step": leaves,
-                   "counts_match_golden": leaves == W.C2_LEAVES},
+                   "counts_match_golden": leaves == W.C2_LEAVES, "tuning": tuning},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
                      "frac": achieved / VALU_PEAK_TOPS,
                      # HBM bytes per launch: FETCH_SIZE x2.00 + WRITE_SIZE x1.00 (rocprofv3 --pmc, calibrated

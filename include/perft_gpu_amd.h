@@ -131,6 +131,14 @@ pf_status pf_set_backend(pf_ctx *ctx, int backend);
  * default): the compile-time FIND_DUPLICATES_IN_BFS of switches.h:50-52 as a
  * run-time switch.  Counts are the same either way. */
 pf_status pf_set_dedup(pf_ctx *ctx, int on);
+/* Autotuner (replaces the reference's compile-time launch shape, perft_bb.h:327-329,
+ * switches.h:14-21, and its launch-depth estimate, launcher.h:1535-1563): times
+ * the current backend on `sample` (n positions, `depth`) over the persistent
+ * kernel's tile shapes and workgroups per CU and, with a table, the TT expand
+ * kernel's workgroups per CU; keeps the fastest and, if `path` is not NULL,
+ * writes every backend's choice there (the file pf_config.tune_file loads).
+ * Counts are checked identical across all tried shapes (PF_EDEVICE if not). */
+pf_status pf_tune(pf_ctx *ctx, const pf_pos48 *sample, int n, int depth, const char *path);
 pf_status pf_get_stats(pf_ctx *ctx, pf_stats *out);
 /* Wait for all device work of this context (the reference's cudaDeviceSynchronize
  * calls around its launches, launcher.h:1490-1492). */

@@ -69,7 +69,7 @@ _lib = None
 # every symbol the C ABI declares (checked by the CPU test suite)
 EXPORTS = ["pf_init", "pf_destroy", "pf_strerror", "pf_fen_to_pos", "pf_pos_to_fen", "pf_perft",
            "pf_perft_batch", "pf_perft_batch_mode", "pf_divide", "pf_move_to_uci", "pf_apply_uci",
-           "pf_set_backend", "pf_set_dedup", "pf_get_stats", "pf_reset_stats", "pf_sync", "pf_probe", "pf_playouts",
+           "pf_set_backend", "pf_set_dedup", "pf_tune", "pf_get_stats", "pf_reset_stats", "pf_sync", "pf_probe", "pf_playouts",
            "pf_count_moves_host", "pf_serve_start", "pf_submit", "pf_poll", "pf_serve_stop", "pf_now"]
 
 
@@ -107,6 +107,7 @@ def load_library(path=LIB_PATH):
                                        ctypes.c_int, ctypes.c_void_p]),
         "pf_count_moves_host": (ctypes.c_int, [P(Pos48)]),
         "pf_set_dedup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "pf_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
         "pf_submit": (ctypes.c_int, [ctypes.c_void_p, P(Pos48), ctypes.c_int, ctypes.c_uint64]),
         "pf_poll": (ctypes.c_int, [ctypes.c_void_p, P(Result), ctypes.c_int]),
         "pf_serve_start": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64]),
@@ -207,6 +208,13 @@ class Engine:
     def set_backend(self, backend):
         _check(self._lib.pf_set_backend(self._ctx, BACKENDS[backend] if isinstance(backend, str) else int(backend)),
                "set_backend")
+
+    def tune(self, positions, depth, path=None):
+        """Autotune the current backend's launch parameters on a sample
+        (pf_tune); optionally write the tuning file."""
+        arr = positions_array(positions)
+        _check(self._lib.pf_tune(self._ctx, arr.ctypes.data, len(arr), depth,
+                                 path.encode() if path else None), "tune")
 
     def set_dedup(self, on):
         _check(self._lib.pf_set_dedup(self._ctx, 1 if on else 0), "set_dedup")

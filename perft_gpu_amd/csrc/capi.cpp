@@ -74,7 +74,7 @@ pf_status pf_init(const pf_config* cfg, pf_ctx** out) {
     if (!ctx) return PF_ENOMEM;
     pf_status s = ctx->eng.init(c);
     if (s == PF_OK) s = ctx->eng.init_tt(c.tt_bytes, c.zobrist_seed);
-    if (s == PF_OK && c.tune_file) ctx->eng.load_tuning(c.tune_file);
+    if (s == PF_OK && c.tune_file) s = ctx->eng.load_tuning(c.tune_file);
     if (s != PF_OK) {
         ctx->eng.destroy();
         delete ctx;
@@ -260,6 +260,15 @@ pf_status pf_set_backend(pf_ctx* ctx, int backend) {
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
     ctx->eng.backend = backend;
     return PF_OK;
+}
+
+pf_status pf_tune(pf_ctx* ctx, const pf_pos48* sample, int n, int depth, const char* path) {
+    if (!ctx || !sample || n <= 0) return PF_EINVAL;
+    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    if (ctx->srv) return PF_EBUSY;
+    pf_status s = ctx->eng.autotune(sample, n, depth);
+    if (s == PF_OK && path) s = ctx->eng.save_tuning(path);
+    return s;
 }
 
 pf_status pf_set_dedup(pf_ctx* ctx, int on) {

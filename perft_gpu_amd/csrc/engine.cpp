@@ -66,10 +66,9 @@ pf_status Engine::init(const pf_config& cfg) {
     PF_HIP(hipEventCreate(&ev0));
     PF_HIP(hipEventCreate(&ev1));
     PF_HIP(hipMalloc(&d_pctl, sizeof(PersistCtl)));
-    for (int b = 0; b < 4; b++) {
+    for (int b = 0; b < 4; b++)
         for (int m = 0; m < 2; m++) grid[b][m] = expand_grid(b, m, n_cu);
-        pgrid[b] = persistent_grid(b, n_cu);
-    }
+    apply_tuning();
     return ensure_sums(1024);
 }
 
@@ -285,7 +284,8 @@ pf_status Engine::perft_persistent(const pf_pos48* pos, int n, int d, u64* out) 
     p.ctl = d_pctl;
     p.sums = d_sums;
     if (timing && hipEventRecord(ev0, stream) != hipSuccess) return fail(PF_EDEVICE);
-    if (launch_persistent(backend, DevTables{d_tab}, p, pgrid[backend], stream) != hipSuccess) return fail(PF_EDEVICE);
+    if (launch_persistent(backend, tune[backend].p_variant, DevTables{d_tab}, p, pgrid[backend], stream) != hipSuccess)
+        return fail(PF_EDEVICE);
     if (timing && hipEventRecord(ev1, stream) != hipSuccess) return fail(PF_EDEVICE);
     st.kernel_launches++;
     PersistCtl h;
@@ -306,7 +306,5 @@ pf_status Engine::perft_persistent(const pf_pos48* pos, int n, int d, u64* out) 
     return PF_OK;
 }
 
-
-void Engine::load_tuning(const char* path) { (void)path; }
 
 }  // namespace pf

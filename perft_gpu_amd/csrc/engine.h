@@ -35,6 +35,14 @@ struct Engine {
 
     int grid[4][2] = {};
     int pgrid[4] = {};
+    // launch parameters per backend, chosen by autotune() or a tune file
+    // (0 blocks per CU = as many as can be resident)
+    struct Tune {
+        int p_variant = 0;  // persistent kernel tile shape (kernels.h)
+        int p_bpc = 0;      // persistent kernel workgroups per CU
+        int tt_bpc = 0;     // TT expand kernel workgroups per CU
+    };
+    Tune tune[4];
     PersistCtl* d_pctl = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     pf_stats st = {};
@@ -76,7 +84,10 @@ struct Engine {
     pf_status perft_levels(const pf_pos48* pos, int n, int depth, u64* out);
     pf_status perft_persistent(const pf_pos48* pos, int n, int depth, u64* out);
     pf_status init_tt(u64 bytes, u64 seed);
-    void load_tuning(const char* path);
+    void apply_tuning();
+    pf_status load_tuning(const char* path);
+    pf_status save_tuning(const char* path) const;
+    pf_status autotune(const pf_pos48* pos, int n, int depth);
     pf_status probe(int kind, u64 bytes, u64 iters, double* out);
 
   private:

@@ -40,7 +40,7 @@ pf_status Engine::init_tt(u64 bytes, u64 seed) {
     PF_HIP(hipMemcpy(d_z, &h_z, sizeof(ZTables), hipMemcpyHostToDevice));
     PF_HIP(hipMalloc(&d_ttstats, 8 * sizeof(unsigned long long)));
     PF_HIP(hipMemset(d_ttstats, 0, 8 * sizeof(unsigned long long)));
-    for (int b = 0; b < 4; b++) tt_grid[b] = expand_tt_grid(b, n_cu);
+    apply_tuning();
     return tt_clear();
 }
 

@@ -56,7 +56,11 @@ struct PersistArgs {
     PersistCtl* ctl;
     unsigned long long* sums;
 };
-int persistent_grid(int backend, int n_cu);
-hipError_t launch_persistent(int backend, const DevTables& dt, const PersistArgs& a, int grid, hipStream_t s);
+// Tile-shape variants of the persistent kernel (kernels.hip, PCfg) and the
+// most workgroups per CU that can be resident for one of them.
+constexpr int kPersistVariants = 3;
+int persistent_max_bpc(int backend, int variant);
+hipError_t launch_persistent(int backend, int variant, const DevTables& dt, const PersistArgs& a, int grid,
+                             hipStream_t s);
 
 }  // namespace pf

@@ -202,6 +202,20 @@ struct Cfg<kByte> {
     static constexpr size_t dyn = kByteLds;
 };
 
+// Tile shapes of the persistent kernel that the tuner chooses between:
+// variant 0 = 256 parents x 8192-entry move window, 1 = 128 x 4096,
+// 2 = 256 x 4096.  The byte-lookup backend keeps its LDS-staged shape.
+template <int B, int V>
+struct PCfg {
+    static constexpr int TP = (V == 1) ? 128 : 256;
+    static constexpr int CAP = (V == 0) ? 8192 : 4096;
+};
+template <int V>
+struct PCfg<kByte, V> {
+    static constexpr int TP = Cfg<kByte>::TP;
+    static constexpr int CAP = Cfg<kByte>::CAP;
+};
+
 template <int B, int MODE>
 __global__ __launch_bounds__(kWG) void k_expand(const SliderTables* tabs, LevelArgs a) {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
@@ -245,12 +259,12 @@ __global__ __launch_bounds__(kWG) void k_range_count(const Board* b, u32 n, unsi
 // signal).  Every workgroup runs every level; grid barriers separate levels.
 // All branch decisions between barriers read the same device words, so every
 // workgroup takes the same path and reaches every barrier.
-template <int B>
+template <int B, int V>
 __global__ __launch_bounds__(kWG) void k_persistent(const SliderTables* tabs, PersistArgs p) {
     namespace cg = cooperative_groups;
     cg::grid_group grid = cg::this_grid();
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    __shared__ TileLds<Cfg<B>::TP, Cfg<B>::CAP> s;
+    __shared__ TileLds<PCfg<B, V>::TP, PCfg<B, V>::CAP> s;
     const auto sl = stage_backend<B>(tabs, dyn_lds);
     PersistCtl* ctl = p.ctl;
 
@@ -322,27 +336,36 @@ static void* pick_expand(int backend, int mode) {
     }
 }
 
-static void* pick_persistent(int backend) {
+template <int B>
+static void* persistent_fn(int variant) {
+    switch (variant) {
+    case 1: return reinterpret_cast<void*>(&k_persistent<B, 1>);
+    case 2: return reinterpret_cast<void*>(&k_persistent<B, 2>);
+    default: return reinterpret_cast<void*>(&k_persistent<B, 0>);
+    }
+}
+
+static void* pick_persistent(int backend, int variant) {
     switch (backend) {
-    case kKS: return reinterpret_cast<void*>(&k_persistent<kKS>);
-    case kPlain: return reinterpret_cast<void*>(&k_persistent<kPlain>);
-    case kFancy: return reinterpret_cast<void*>(&k_persistent<kFancy>);
-    default: return reinterpret_cast<void*>(&k_persistent<kByte>);
+    case kKS: return persistent_fn<kKS>(variant);
+    case kPlain: return persistent_fn<kPlain>(variant);
+    case kFancy: return persistent_fn<kFancy>(variant);
+    default: return reinterpret_cast<void*>(&k_persistent<kByte, 0>);
     }
 }
 
 static size_t dyn_bytes(int backend) { return backend == kByte ? Cfg<kByte>::dyn : 0; }
 
-static int occupancy_grid(void* f, int backend, int n_cu) {
+static int blocks_per_cu(void* f, int backend) {
     if (backend == kByte) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_bytes(backend));
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kWG, dyn_bytes(backend)) != hipSuccess || per_cu < 1)
         per_cu = 1;
-    return per_cu * n_cu;
+    return per_cu;
 }
 
-int expand_grid(int backend, int mode, int n_cu) { return occupancy_grid(pick_expand(backend, mode), backend, n_cu); }
-int persistent_grid(int backend, int n_cu) { return occupancy_grid(pick_persistent(backend), backend, n_cu); }
+int expand_grid(int backend, int mode, int n_cu) { return blocks_per_cu(pick_expand(backend, mode), backend) * n_cu; }
+int persistent_max_bpc(int backend, int variant) { return blocks_per_cu(pick_persistent(backend, variant), backend); }
 
 hipError_t launch_init_roots(int backend, const DevTables& dt, Board* roots, u32 n,
                              unsigned long long* sums, hipStream_t s) {
@@ -366,8 +389,9 @@ hipError_t launch_expand(int backend, int mode, const DevTables& dt, const Level
     return hipLaunchKernel(f, dim3(grid), dim3(kWG), kargs, dyn_bytes(backend), s);
 }
 
-hipError_t launch_persistent(int backend, const DevTables& dt, const PersistArgs& a, int grid, hipStream_t s) {
-    void* f = pick_persistent(backend);
+hipError_t launch_persistent(int backend, int variant, const DevTables& dt, const PersistArgs& a, int grid,
+                             hipStream_t s) {
+    void* f = pick_persistent(backend, variant);
     PersistArgs args = a;
     const SliderTables* t = dt.t;
     void* kargs[] = {(void*)&t, (void*)&args};

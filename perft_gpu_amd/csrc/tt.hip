@@ -491,11 +491,11 @@ static void* pick(int backend) {
     }
 }
 
-int expand_tt_grid(int backend, int n_cu) {
+int expand_tt_max_bpc(int backend) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pick(backend), kTWG, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
-    return per_cu * n_cu;
+    return per_cu;
 }
 
 hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables* zt, const TTArgs& a, int grid,

@@ -137,7 +137,7 @@ struct StepTable {
 int step_grid(int backend, int n_cu);
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
                        int grid, int up_grid, hipStream_t s);
-int expand_tt_grid(int backend, int n_cu);
+int expand_tt_max_bpc(int backend);  // most resident workgroups per CU of the TT expand kernel
 hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables* zt, const TTArgs& a, int grid,
                             hipStream_t s);
 hipError_t launch_roots_tt(int backend, const SliderTables* tabs, Board* roots, const u64* k1, const u64* k2, u32 n,

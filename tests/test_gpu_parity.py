@@ -131,3 +131,31 @@ def test_divide(pg, oracle, engine):
     for m, c in oracle.divide(o, 4):
         want[pg.move_to_uci(m & 0x0FFF | ((m >> 12) & 0xB) << 12 if (m >> 12) & 8 else m & 0x0FFF)] = c
     assert dict(div) == want
+
+
+def test_autotune_keeps_counts_and_round_trips(pg, oracle, tmp_path):
+    """the tuner tries every tile shape / workgroups-per-CU choice (each must
+    give the same counts), writes its choice, and a context initialised from
+    the file gives the same counts"""
+    pos = pg.playouts(7070, 32, 8, 40)
+    want = oracle.perft_batch(pos, 4, nthreads=8)
+    path = str(tmp_path / "tune.txt")
+    eng = pg.Engine(backend="ks", arena_bytes=2 << 30, tt_bytes=64 << 20)
+    try:
+        eng.tune(pos, 4, path)
+        assert np.array_equal(eng.perft_batch(pos, 4, mode=pg.MODE_PERSISTENT), want)
+        assert np.array_equal(eng.perft_batch(pos, 4), want)
+    finally:
+        eng.close()
+    text = open(path).read()
+    assert text.startswith("perft_gpu_amd tuning 1") and "backend ks persistent_variant" in text
+    eng = pg.Engine(backend="ks", arena_bytes=2 << 30, tune_file=path)
+    try:
+        assert np.array_equal(eng.perft_batch(pos, 4, mode=pg.MODE_PERSISTENT), want)
+    finally:
+        eng.close()
+    bad = tmp_path / "bad.txt"
+    bad.write_text("not a tuning file\n")
+    with pytest.raises(pg.PerftError) as ei:
+        pg.Engine(backend="ks", arena_bytes=1 << 30, tune_file=str(bad))
+    assert ei.value.status == pg.PF_EINVAL
