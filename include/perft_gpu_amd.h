@@ -41,8 +41,11 @@ typedef enum pf_status {
     PF_EINVAL = -1,   /* bad argument, illegal FEN, depth out of range */
     PF_ENOMEM = -2,   /* device memory exhausted (reference: ALLSET sentinel) */
     PF_EDEVICE = -3,  /* HIP error or no usable GPU */
-    PF_EBUSY = -4,    /* serving queue full */
-    PF_EAGAIN = -5    /* nothing to report yet */
+    PF_EBUSY = -4,    /* serving queue full, or the context is serving */
+    PF_EAGAIN = -5,   /* nothing to report yet */
+    PF_EUNAVAIL = -6  /* PF_MODE_PERSISTENT: the device refused the cooperative launch
+                         (its grid-wide barrier resource is in use, e.g. by another
+                         process); PF_MODE_AUTO falls back to the level driver */
 } pf_status;
 
 typedef enum pf_backend {
@@ -139,6 +142,17 @@ pf_status pf_set_dedup(pf_ctx *ctx, int on);
  * writes every backend's choice there (the file pf_config.tune_file loads).
  * Counts are checked identical across all tried shapes (PF_EDEVICE if not). */
 pf_status pf_tune(pf_ctx *ctx, const pf_pos48 *sample, int n, int depth, const char *path);
+
+/* Records mode (replaces processPerftRecords, launcher.h:1578-1695): perft(depth)
+ * of every record "<FEN ...> <occurrences>" of the text file `in_path`, appended
+ * to "<in_path>.op" as "<record> <perft> <perft * occurrences>" in input order,
+ * `batch` records per pf_perft_batch call (0 = 256).  Records that already
+ * have an output line are skipped, so an interrupted run resumes; a torn last
+ * output line is dropped first.  *weighted_total = the sum of the last column
+ * over the whole output file; *records_done = records evaluated by this call.
+ * PF_EINVAL for an unreadable record or an output file of another input. */
+pf_status pf_records(pf_ctx *ctx, const char *in_path, int depth, int batch, uint64_t *weighted_total,
+                     uint64_t *records_done);
 pf_status pf_get_stats(pf_ctx *ctx, pf_stats *out);
 /* Wait for all device work of this context (the reference's cudaDeviceSynchronize
  * calls around its launches, launcher.h:1490-1492). */

@@ -61,6 +61,7 @@ const char* pf_strerror(int s) {
     case PF_EDEVICE: return "device error";
     case PF_EBUSY: return "busy";
     case PF_EAGAIN: return "no result yet";
+    case PF_EUNAVAIL: return "cooperative launch unavailable";
     default: return "unknown status";
     }
 }
@@ -201,7 +202,7 @@ pf_status pf_perft_batch_mode(pf_ctx* ctx, const pf_pos48* pos, int n, int depth
     if (mode == PF_MODE_LEVELS || (mode == PF_MODE_AUTO && ctx->eng.d_tt && depth >= 3))
         return ctx->eng.perft_levels(pos, n, depth, out);
     pf_status s = ctx->eng.perft_persistent(pos, n, depth, out);
-    if (s == PF_ENOMEM && mode == PF_MODE_AUTO) s = ctx->eng.perft_levels(pos, n, depth, out);
+    if ((s == PF_ENOMEM || s == PF_EUNAVAIL) && mode == PF_MODE_AUTO) s = ctx->eng.perft_levels(pos, n, depth, out);
     return s;
 }
 

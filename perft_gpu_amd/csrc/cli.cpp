@@ -11,6 +11,10 @@
 //       setoption name Backend value ks|plain|fancy|byte, quit.
 //       The reference's UciInterface.cpp is compiled out (#if 0) and has no
 //       perft verb; this is the build's own front end.
+//   perft_gpu records <file> <depth> [backend] [tt_mib]
+//       records mode (processPerftRecords, launcher.h:1578-1695): perft of
+//       every "<FEN> <occurrences>" line, appended to <file>.op, resumable;
+//       prints the weighted total.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -145,10 +149,37 @@ static int run_uci() {
     return 0;
 }
 
+static int run_records(int argc, char** argv) {
+    const int depth = atoi(argv[3]);
+    pf_config cfg{};
+    cfg.backend = PF_BACKEND_KS;
+    if (argc > 4) {
+        cfg.backend = backend_of(argv[4]);
+        if (cfg.backend < 0) { fprintf(stderr, "unknown backend %s\n", argv[4]); return 2; }
+    }
+    if (argc > 5) cfg.tt_bytes = (uint64_t)atoll(argv[5]) << 20;
+    pf_ctx* ctx = nullptr;
+    int s = pf_init(&cfg, &ctx);
+    if (s != PF_OK) { fprintf(stderr, "pf_init: %s\n", pf_strerror(s)); return 1; }
+    uint64_t total = 0, done = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    s = pf_records(ctx, argv[2], depth, 0, &total, &done);
+    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    pf_destroy(ctx);
+    if (s != PF_OK) { fprintf(stderr, "records: %s\n", pf_strerror(s)); return 1; }
+    printf("Records done: %llu, Total: %g seconds, weighted perft %d: %llu\n", (unsigned long long)done, el, depth,
+           (unsigned long long)total);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 2 && !strcmp(argv[1], "uci")) return run_uci();
+    if (argc >= 4 && !strcmp(argv[1], "records")) return run_records(argc, argv);
     if (argc < 3) {
-        fprintf(stderr, "usage: perft_gpu <fen|startpos> <depth> [ks|plain|fancy|byte] [tt_mib]\n       perft_gpu uci\n");
+        fprintf(stderr,
+                "usage: perft_gpu <fen|startpos> <depth> [ks|plain|fancy|byte] [tt_mib]\n"
+                "       perft_gpu uci\n"
+                "       perft_gpu records <file> <depth> [ks|plain|fancy|byte] [tt_mib]\n");
         return 2;
     }
     return run_cli(argc, argv);

@@ -284,8 +284,12 @@ pf_status Engine::perft_persistent(const pf_pos48* pos, int n, int d, u64* out) 
     p.ctl = d_pctl;
     p.sums = d_sums;
     if (timing && hipEventRecord(ev0, stream) != hipSuccess) return fail(PF_EDEVICE);
-    if (launch_persistent(backend, tune[backend].p_variant, DevTables{d_tab}, p, pgrid[backend], stream) != hipSuccess)
-        return fail(PF_EDEVICE);
+    if (launch_persistent(backend, tune[backend].p_variant, DevTables{d_tab}, p, pgrid[backend], stream) !=
+        hipSuccess) {
+        // refused before it started (nothing ran): clear the error for the caller's fallback
+        (void)hipGetLastError();
+        return fail(PF_EUNAVAIL);
+    }
     if (timing && hipEventRecord(ev1, stream) != hipSuccess) return fail(PF_EDEVICE);
     st.kernel_launches++;
     PersistCtl h;

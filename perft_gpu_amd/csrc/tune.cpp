@@ -112,7 +112,7 @@ pf_status Engine::autotune(const pf_pos48* pos, int n, int d) {
             apply_tuning();
             double ms = 0;
             s = time_persistent(&ms);
-            if (s == PF_ENOMEM) {  // the sample does not fit the persistent kernel
+            if (s == PF_ENOMEM || s == PF_EUNAVAIL) {  // the sample does not fit, or no cooperative launch now
                 s = PF_OK;
                 break;
             }
