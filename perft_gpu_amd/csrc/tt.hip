@@ -24,6 +24,10 @@ namespace pf {
 constexpr int kTWG = 256;
 constexpr int kTTP = 128;   // parents per tile
 constexpr int kTCAP = 4096; // LDS move-list window
+// The write kernels wait on random table and duplicate-claim traffic most of
+// the time: ask for at least this many resident waves per SIMD (register
+// budget 512 / waves) so that more of that latency is hidden.
+constexpr int kTTWaves = 4;
 
 __device__ __forceinline__ u32 t_wave_incl_scan(u32 v, int lane) {
 #pragma unroll
@@ -303,7 +307,7 @@ __device__ void expand_tt_level(const SL& sl, const TTArgs& a, const ZTables* zt
 }
 
 template <int B>
-__global__ __launch_bounds__(kTWG) void k_expand_tt(const SliderTables* tabs, const ZTables* zt, TTArgs a) {
+__global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(kTTWaves))) void k_expand_tt(const SliderTables* tabs, const ZTables* zt, TTArgs a) {
     __shared__ TTLds s;
     const auto sl = SlOfT<B>::make(tabs);
     expand_tt_level(sl, a, zt, s);
@@ -366,7 +370,7 @@ __global__ __launch_bounds__(256) void k_upsweep(UpSeg u, uint4* tt, u64 mask, u
 // block copies its segment's arguments to LDS and runs the tile.  Segments of
 // different jobs touch disjoint memory except the lockless table.
 template <int B>
-__global__ __launch_bounds__(kTWG) void k_step(const SliderTables* tabs, const ZTables* zt, StepTable* t) {
+__global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(kTTWaves))) void k_step(const SliderTables* tabs, const ZTables* zt, StepTable* t) {
     __shared__ TTLds s;
     __shared__ TTArgs sa;
     __shared__ u32 s_tile, s_seg;
