@@ -79,14 +79,24 @@ __device__ __forceinline__ u64 dup_slot(u64 k1, u64 n) { return ((k1 >> 32) * n)
 // is written and publishes its index there), 1 when an earlier child with
 // its keys holds *slot (a duplicate), 2 when the probe sequence is full or
 // the key is 0 (written, no duplicate detection).
+// A slot goes from 0 to a key once in a level's life, so a non-zero key read
+// at device scope is final: a duplicate (most children at deep levels) is
+// recognised by a load, and only an empty slot needs the compare-and-swap.
 __device__ __forceinline__ int dup_claim(DupEntry* t, u64 n, u64 k1, u64 k2, u64* slot) {
     if (!k2) return 2;
     u64 s = dup_slot(k1, n);
     for (int p = 0; p < kDupProbes; p++) {
-        const unsigned long long old = atomicCAS(&t[s].key, 0ull, (unsigned long long)k2);
-        if (old == 0 || old == k2) {
+        unsigned long long cur = __hip_atomic_load(&t[s].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == 0) {
+            cur = atomicCAS(&t[s].key, 0ull, (unsigned long long)k2);
+            if (cur == 0) {
+                *slot = s;
+                return 0;
+            }
+        }
+        if (cur == k2) {
             *slot = s;
-            return old == 0 ? 0 : 1;
+            return 1;
         }
         s = (s + 1 == n) ? 0 : s + 1;
     }
