@@ -133,7 +133,8 @@ u64 Server::predict_bytes(const pf_pos48& pos, int depth) const {
     const u64 c = count_moves(to_board(pos), KSBackend{});
     const double bf = (double)std::min<u64>(48, std::max<u64>(16, c));
     // (duplicate elimination makes the deep levels smaller, but a region that
-    // proves too small costs many chunked steps, so no credit is taken for it)
+    // proves too small costs a chain of chunked steps at every level below,
+    // so no credit is taken for it)
     double lvl = (double)c, sum = 0;
     for (int k = 1; k <= depth - 2; k++) {
         sum += lvl;
