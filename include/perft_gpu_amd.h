@@ -114,7 +114,13 @@ pf_status pf_perft(pf_ctx *ctx, const pf_pos48 *pos, int depth, uint64_t *out);
 
 /* Batched perft of n positions (any mix of sides to move): replaces
  * perft_bb_gpu_simple_hash(count, positions, ..., perfts, depth, ...)
- * (perft_bb.h:1242), which needs one side to move and <= 256 roots. */
+ * (perft_bb.h:1242), which needs one side to move and <= 256 roots.
+ * `pos` and `out` may each be host memory or device memory of the context's
+ * device (as in the reference, whose kernel takes device arrays).  With both
+ * on the device, a table-free run (PF_MODE_PERSISTENT, or AUTO with no table)
+ * converts the roots on the device and leaves the counts there: no host
+ * staging.  Other combinations stage through host memory.  A device pointer
+ * of another device is PF_EINVAL. */
 pf_status pf_perft_batch(pf_ctx *ctx, const pf_pos48 *pos, int n, int depth, uint64_t *out);
 pf_status pf_perft_batch_mode(pf_ctx *ctx, const pf_pos48 *pos, int n, int depth, int mode, uint64_t *out);
 

@@ -176,6 +176,72 @@ def playouts(seed, n, min_plies, max_plies, min_moves=0, no_check=False):
     return arr
 
 
+_hip_rt = None
+
+
+def _hip():
+    """The HIP runtime the engine library itself links (same loaded copy)."""
+    global _hip_rt
+    if _hip_rt is None:
+        load_library()
+        rt = ctypes.CDLL("libamdhip64.so.7")
+        for name, args in (("hipSetDevice", [ctypes.c_int]),
+                           ("hipMalloc", [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]),
+                           ("hipFree", [ctypes.c_void_p]),
+                           ("hipMemcpy", [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int])):
+            fn = getattr(rt, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = args
+        _hip_rt = rt
+    return _hip_rt
+
+
+def _hip_check(err, what):
+    if err != 0:
+        raise RuntimeError("%s: HIP error %d" % (what, err))
+
+
+class DeviceArray:
+    """n elements of a numpy dtype in device memory (hipMalloc), for batches
+    that stay resident in HBM between calls; `ptr` is the device address."""
+
+    def __init__(self, n, dtype, device=0):
+        self.dtype, self.n, self.device = np.dtype(dtype), int(n), device
+        self.nbytes = self.n * self.dtype.itemsize
+        hip, p = _hip(), ctypes.c_void_p()
+        _hip_check(hip.hipSetDevice(device), "hipSetDevice")
+        _hip_check(hip.hipMalloc(ctypes.byref(p), max(self.nbytes, 1)), "hipMalloc(%d)" % self.nbytes)
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, arr, device=0):
+        arr = np.ascontiguousarray(arr)
+        d = cls(len(arr), arr.dtype, device)
+        d.upload(arr)
+        return d
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes == self.nbytes
+        _hip_check(_hip().hipMemcpy(self.ptr, arr.ctypes.data, self.nbytes, 1), "hipMemcpy H2D")
+
+    def download(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        _hip_check(_hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2), "hipMemcpy D2H")
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            _hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Engine:
     """One device context (pf_ctx).  Mirrors the reference's GPU perft entry
     points; every compute call runs the HIP kernels."""
@@ -238,6 +304,14 @@ class Engine:
         _check(self._lib.pf_perft_batch_mode(self._ctx, arr.ctypes.data_as(ctypes.c_void_p), len(arr), depth, mode,
                                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))), "perft_batch")
         return out
+
+    def perft_batch_ptr(self, pos_ptr, n, depth, out_ptr, mode=MODE_AUTO):
+        """pf_perft_batch_mode on raw addresses: `pos_ptr` -> n pf_pos48 records,
+        `out_ptr` -> n uint64 counts, each host or device memory of this
+        context's device (e.g. a torch tensor's data_ptr())."""
+        _check(self._lib.pf_perft_batch_mode(self._ctx, ctypes.c_void_p(pos_ptr), n, depth, mode,
+                                             ctypes.cast(ctypes.c_void_p(out_ptr), ctypes.POINTER(ctypes.c_uint64))),
+               "perft_batch_ptr")
 
     def divide(self, pos, depth):
         mv = (ctypes.c_uint16 * 256)()

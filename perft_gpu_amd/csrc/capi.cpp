@@ -193,16 +193,62 @@ pf_status pf_perft(pf_ctx* ctx, const pf_pos48* pos, int depth, uint64_t* out) {
     return ctx->eng.perft_levels(pos, 1, depth, out);
 }
 
-pf_status pf_perft_batch_mode(pf_ctx* ctx, const pf_pos48* pos, int n, int depth, int mode, uint64_t* out) {
-    if (!ctx || (n > 0 && (!pos || !out)) || n < 0 || check_depth(depth)) return PF_EINVAL;
-    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
-    if (ctx->eng.serving) return PF_EBUSY;
+// Where a caller's buffer lives: 0 host (pageable, pinned or managed), 1 device
+// memory of `device`, -1 device memory of another device.
+static int buffer_kind(const void* p, int device) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory the runtime has never seen
+        return 0;
+    }
+    if (a.type != hipMemoryTypeDevice) return 0;
+    return a.device == device ? 1 : -1;
+}
+
+static pf_status batch_host(pf_ctx* ctx, const pf_pos48* pos, int n, int depth, int mode, uint64_t* out) {
     // the transposition table lives in the level driver; AUTO uses it when
     // configured, PERSISTENT always runs the single table-free launch
     if (mode == PF_MODE_LEVELS || (mode == PF_MODE_AUTO && ctx->eng.d_tt && depth >= 3))
         return ctx->eng.perft_levels(pos, n, depth, out);
     pf_status s = ctx->eng.perft_persistent(pos, n, depth, out);
     if ((s == PF_ENOMEM || s == PF_EUNAVAIL) && mode == PF_MODE_AUTO) s = ctx->eng.perft_levels(pos, n, depth, out);
+    return s;
+}
+
+pf_status pf_perft_batch_mode(pf_ctx* ctx, const pf_pos48* pos, int n, int depth, int mode, uint64_t* out) {
+    if (!ctx || (n > 0 && (!pos || !out)) || n < 0 || check_depth(depth)) return PF_EINVAL;
+    if (mode != PF_MODE_AUTO && mode != PF_MODE_LEVELS && mode != PF_MODE_PERSISTENT) return PF_EINVAL;
+    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    if (ctx->eng.serving) return PF_EBUSY;
+    if (n == 0) return PF_OK;
+    if (hipSetDevice(ctx->eng.device) != hipSuccess) return PF_EDEVICE;
+    const int kin = buffer_kind(pos, ctx->eng.device), kout = buffer_kind(out, ctx->eng.device);
+    if (kin < 0 || kout < 0) return PF_EINVAL;
+    if (!kin && !kout) return batch_host(ctx, pos, n, depth, mode, out);
+    const bool table_run = mode == PF_MODE_LEVELS || (mode == PF_MODE_AUTO && ctx->eng.d_tt && depth >= 3);
+    if (kin && kout && !table_run && depth >= 2) {
+        pf_status s = ctx->eng.perft_persistent(pos, n, depth, out, true, true);
+        if (!((s == PF_ENOMEM || s == PF_EUNAVAIL) && mode == PF_MODE_AUTO)) return s;
+    }
+    // staged: device buffers round-trip through host memory around the host path
+    std::vector<pf_pos48> hp;
+    std::vector<uint64_t> ho;
+    const pf_pos48* p = pos;
+    uint64_t* o = out;
+    if (kin) {
+        hp.resize((size_t)n);
+        if (hipMemcpy(hp.data(), pos, (size_t)n * sizeof(pf_pos48), hipMemcpyDeviceToHost) != hipSuccess)
+            return PF_EDEVICE;
+        p = hp.data();
+    }
+    if (kout) {
+        ho.resize((size_t)n);
+        o = ho.data();
+    }
+    pf_status s = batch_host(ctx, p, n, depth, mode, o);
+    if (s == PF_OK && kout &&
+        hipMemcpy(out, ho.data(), (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice) != hipSuccess)
+        return PF_EDEVICE;
     return s;
 }
 

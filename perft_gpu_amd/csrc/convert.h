@@ -6,9 +6,9 @@
 
 namespace pf {
 
-inline int pos_black_to_move(const pf_pos48& p) { return (int)(p.pawns >> 63); }
+PF_HD int pos_black_to_move(const pf_pos48& p) { return (int)(p.pawns >> 63); }
 
-inline Board to_board(const pf_pos48& p) {
+PF_HD Board to_board(const pf_pos48& p) {
     const u64 pawns = p.pawns & kRanks2to7;
     const u64 occ = pawns | p.knights | p.bishop_queens | p.rook_queens | p.kings;
     const int wc = (int)(p.pawns & 3), bc = (int)((p.pawns >> 2) & 3), ep = (int)((p.pawns >> 4) & 15);
@@ -31,7 +31,7 @@ inline Board to_board(const pf_pos48& p) {
     return b;
 }
 
-inline pf_pos48 to_pos48(const Board& b, int black) {
+PF_HD pf_pos48 to_pos48(const Board& b, int black) {
     const u64 pawns = b.pawns & kRanks2to7;
     const u64 occ = pawns | b.knights | b.bishops | b.rooks | b.kings;
     const int r = (int)(b.pawns & 15), ep = (int)((b.pawns >> 4) & 15);

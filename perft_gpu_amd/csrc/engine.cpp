@@ -205,10 +205,16 @@ pf_status Engine::descend(const Board* lvl, const u32* rts, u64 n, u64 n_next, i
     return PF_OK;
 }
 
-pf_status Engine::upload_roots(const pf_pos48* pos, int n, Board** roots, u32** rid) {
+pf_status Engine::upload_roots(const pf_pos48* pos, int n, Board** roots, u32** rid, bool dev_in) {
     *roots = reinterpret_cast<Board*>(alloc((u64)n * sizeof(Board)));
     *rid = reinterpret_cast<u32*>(alloc((u64)n * sizeof(u32)));
     if (!*roots || !*rid) return PF_ENOMEM;
+    if (dev_in) {  // already resident: convert in place on the stream, no host round trip
+        if (launch_pos_to_board(pos, (u32)n, *roots, *rid, stream) != hipSuccess) return PF_EDEVICE;
+        st.kernel_launches++;
+        PF_HIP(hipMemsetAsync(d_sums, 0, (size_t)n * 8, stream));
+        return PF_OK;
+    }
     std::vector<Board> hb((size_t)n);
     std::vector<u32> hr((size_t)n);
     for (int i = 0; i < n; i++) { hb[(size_t)i] = to_board(pos[i]); hr[(size_t)i] = (u32)i; }
@@ -261,9 +267,12 @@ pf_status Engine::perft_levels(const pf_pos48* pos, int n, int d, u64* out) {
     return PF_OK;
 }
 
-pf_status Engine::perft_persistent(const pf_pos48* pos, int n, int d, u64* out) {
+pf_status Engine::perft_persistent(const pf_pos48* pos, int n, int d, u64* out, bool dev_in, bool dev_out) {
     if (n <= 0) return PF_OK;
-    if (d <= 1) return perft_levels(pos, n, d, out);
+    if (d <= 1) {
+        if (dev_in || dev_out) return PF_EINVAL;  // callers stage shallow device batches themselves
+        return perft_levels(pos, n, d, out);
+    }
     PF_HIP(hipSetDevice(device));
     pf_status s = ensure_sums((u64)n);
     if (s != PF_OK) return s;
@@ -271,7 +280,7 @@ pf_status Engine::perft_persistent(const pf_pos48* pos, int n, int d, u64* out) 
     auto fail = [&](pf_status e) { release(mark); return e; };
     Board* roots = nullptr;
     u32* rid = nullptr;
-    s = upload_roots(pos, n, &roots, &rid);
+    s = upload_roots(pos, n, &roots, &rid, dev_in);
     if (s != PF_OK) return fail(s);
     PF_HIP(hipMemsetAsync(d_pctl, 0, sizeof(PersistCtl), stream));
     PersistArgs p;
@@ -294,7 +303,8 @@ pf_status Engine::perft_persistent(const pf_pos48* pos, int n, int d, u64* out) 
     st.kernel_launches++;
     PersistCtl h;
     if (hipMemcpyAsync(&h, d_pctl, sizeof(PersistCtl), hipMemcpyDeviceToHost, stream) != hipSuccess ||
-        hipMemcpyAsync(out, d_sums, (size_t)n * 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipMemcpyAsync(out, d_sums, (size_t)n * 8, dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                       stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess)
         return fail(PF_EDEVICE);
     if (timing) {

@@ -82,7 +82,9 @@ struct Engine {
     void destroy();
     pf_status ensure_sums(u64 n);
     pf_status perft_levels(const pf_pos48* pos, int n, int depth, u64* out);
-    pf_status perft_persistent(const pf_pos48* pos, int n, int depth, u64* out);
+    // dev_in / dev_out: `pos` / `out` are device memory of this engine's device
+    pf_status perft_persistent(const pf_pos48* pos, int n, int depth, u64* out, bool dev_in = false,
+                               bool dev_out = false);
     pf_status init_tt(u64 bytes, u64 seed);
     void apply_tuning();
     pf_status load_tuning(const char* path);
@@ -96,7 +98,7 @@ struct Engine {
     pf_status expand(int mode, const Board* in, const u32* in_root, u64 n_in, Board* out, u32* out_root,
                      u64 out_cap, unsigned long long* sums, u64* next_total);
     pf_status range_count(const Board* b, u64 n, u64* total);
-    pf_status upload_roots(const pf_pos48* pos, int n, Board** roots, u32** rid);
+    pf_status upload_roots(const pf_pos48* pos, int n, Board** roots, u32** rid, bool dev_in = false);
     pf_status descend(const Board* lvl, const u32* rts, u64 n, u64 n_next, int L);
     bool alloc_tlevel(u64 n, TLevel* lv);
     pf_status expand_tt(int mode, const TLevel& lv, u64 off, u64 m, TLevel* nx, u64 cap, int L, u64* written,

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "convert.h"
 #include "movegen.h"
 
 namespace pf {
@@ -36,6 +37,8 @@ hipError_t launch_init_roots(int backend, const DevTables& dt, Board* roots, u32
 hipError_t launch_expand(int backend, int mode, const DevTables& dt, const LevelArgs& a,
                          int grid, hipStream_t s);
 hipError_t launch_range_count(const Board* b, u32 n, unsigned long long* out, hipStream_t s);
+// Roots already in device memory (reference layout) -> boards + root ids.
+hipError_t launch_pos_to_board(const pf_pos48* in, u32 n, Board* out, u32* rid, hipStream_t s);
 
 // Whole-batch breadth-first perft in one cooperative launch (grid barrier
 // between levels).  Levels 1..depth-2 are bump-allocated in `arena`.

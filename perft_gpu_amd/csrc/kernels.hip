@@ -238,6 +238,23 @@ __global__ __launch_bounds__(kWG) void k_init_roots(const SliderTables* t, Board
     }
 }
 
+// Device-resident roots in the reference layout -> side-to-move-frame boards
+// and root ids (the batch may start from positions already in HBM, as the
+// reference's perft_bb_gpu_simple_hash does).
+__global__ __launch_bounds__(kWG) void k_pos_to_board(const pf_pos48* in, u32 n, Board* out, u32* rid) {
+    const u32 i = blockIdx.x * kWG + threadIdx.x;
+    if (i < n) {
+        out[i] = to_board(in[i]);
+        rid[i] = i;
+    }
+}
+
+hipError_t launch_pos_to_board(const pf_pos48* in, u32 n, Board* out, u32* rid, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_pos_to_board, dim3((n + kWG - 1) / kWG), dim3(kWG), 0, s, in, n, out, rid);
+    return hipGetLastError();
+}
+
 // Sum of the move counts stored in boards [0, n): the exact size of the next
 // level for a chunk of this one.
 __global__ __launch_bounds__(kWG) void k_range_count(const Board* b, u32 n, unsigned long long* out) {

@@ -159,3 +159,48 @@ def test_autotune_keeps_counts_and_round_trips(pg, oracle, tmp_path):
     with pytest.raises(pg.PerftError) as ei:
         pg.Engine(backend="ks", arena_bytes=1 << 30, tune_file=str(bad))
     assert ei.value.status == pg.PF_EINVAL
+
+
+def test_device_resident_batches(pg, oracle, engine):
+    """positions and counts in device memory (pf_perft_batch on device
+    pointers): the persistent path converts on the device, every other
+    combination stages through the host; all give the oracle's counts"""
+    pos = pg.playouts(5150, 64, 8, 50)
+    want = {d: oracle.perft_batch(pos, d, nthreads=8) for d in (1, 2, 4)}
+    dpos = pg.DeviceArray.from_host(pos)
+    dout = pg.DeviceArray(len(pos), np.uint64)
+    hout = np.zeros(len(pos), dtype=np.uint64)
+    try:
+        for mode in (pg.MODE_AUTO, pg.MODE_PERSISTENT, pg.MODE_LEVELS):
+            for d in (1, 2, 4):
+                dout.upload(np.full(len(pos), 7, dtype=np.uint64))
+                engine.perft_batch_ptr(dpos.ptr, len(pos), d, dout.ptr, mode)
+                assert np.array_equal(dout.download(), want[d]), (mode, d)
+        # device positions, host counts; host positions, device counts
+        engine.perft_batch_ptr(dpos.ptr, len(pos), 4, hout.ctypes.data, pg.MODE_PERSISTENT)
+        assert np.array_equal(hout, want[4])
+        engine.perft_batch_ptr(pos.ctypes.data, len(pos), 4, dout.ptr, pg.MODE_PERSISTENT)
+        assert np.array_equal(dout.download(), want[4])
+        # the subset at an offset inside the device buffer
+        engine.perft_batch_ptr(dpos.ptr + 48 * 10, 20, 4, dout.ptr, pg.MODE_PERSISTENT)
+        assert np.array_equal(dout.download()[:20], want[4][10:30])
+    finally:
+        dpos.free()
+        dout.free()
+
+
+def test_device_resident_batch_with_table(pg, oracle):
+    """with a transposition table AUTO runs the level driver: device buffers
+    are staged, counts unchanged"""
+    pos = pg.playouts(5151, 24, 8, 50)
+    want = oracle.perft_batch(pos, 4, nthreads=8)
+    eng = pg.Engine(backend="ks", arena_bytes=1 << 30, tt_bytes=64 << 20)
+    dpos = pg.DeviceArray.from_host(pos)
+    dout = pg.DeviceArray(len(pos), np.uint64)
+    try:
+        eng.perft_batch_ptr(dpos.ptr, len(pos), 4, dout.ptr, pg.MODE_AUTO)
+        assert np.array_equal(dout.download(), want)
+    finally:
+        dpos.free()
+        dout.free()
+        eng.close()
