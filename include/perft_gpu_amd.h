@@ -43,9 +43,10 @@ typedef enum pf_status {
     PF_EDEVICE = -3,  /* HIP error or no usable GPU */
     PF_EBUSY = -4,    /* serving queue full, or the context is serving */
     PF_EAGAIN = -5,   /* nothing to report yet */
-    PF_EUNAVAIL = -6  /* PF_MODE_PERSISTENT: the device refused the cooperative launch
+    PF_EUNAVAIL = -6, /* PF_MODE_PERSISTENT: the device refused the cooperative launch
                          (its grid-wide barrier resource is in use, e.g. by another
                          process); PF_MODE_AUTO falls back to the level driver */
+    PF_EIO = -7       /* a file could not be opened, read or written */
 } pf_status;
 
 typedef enum pf_backend {
@@ -159,6 +160,22 @@ pf_status pf_tune(pf_ctx *ctx, const pf_pos48 *sample, int n, int depth, const c
  * PF_EINVAL for an unreadable record or an output file of another input. */
 pf_status pf_records(pf_ctx *ctx, const char *in_path, int depth, int batch, uint64_t *weighted_total,
                      uint64_t *records_done);
+
+/* Persistent transposition table (replaces the reference's disk hash,
+ * diskTTProbe / diskTTStore and perfthash.dat, launcher.h:700-789, which keeps
+ * deep subtree counts across runs).  pf_tt_save writes the context's whole
+ * table to `path` (a 64-byte header, then the buckets as they lie in HBM).
+ * pf_tt_load replaces the table's contents with a saved one: a table of the
+ * same size is restored exactly; a larger one folds into this one keeping
+ * each bucket's two deepest entries, a smaller one is replicated to every
+ * bucket its keys can map to.  Entries stay verified by their second key, so
+ * a restored table changes no count.  PF_EINVAL: no table in this context, not
+ * a table file, a truncated file, or a table made with other Zobrist keys
+ * (pf_config.zobrist_seed); PF_EIO: the file cannot be opened, read or
+ * written; PF_EBUSY while serving. */
+pf_status pf_tt_save(pf_ctx *ctx, const char *path);
+pf_status pf_tt_load(pf_ctx *ctx, const char *path);
+
 pf_status pf_get_stats(pf_ctx *ctx, pf_stats *out);
 /* Wait for all device work of this context (the reference's cudaDeviceSynchronize
  * calls around its launches, launcher.h:1490-1492). */

@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libperft_gpu_amd.so")
 CLI_PATH = os.path.join(HERE, "_build", "perft_gpu")
 
-PF_OK, PF_EINVAL, PF_ENOMEM, PF_EDEVICE, PF_EBUSY, PF_EAGAIN, PF_EUNAVAIL = 0, -1, -2, -3, -4, -5, -6
+PF_OK, PF_EINVAL, PF_ENOMEM, PF_EDEVICE, PF_EBUSY, PF_EAGAIN, PF_EUNAVAIL, PF_EIO = 0, -1, -2, -3, -4, -5, -6, -7
 BACKENDS = {"ks": 0, "plain": 1, "fancy": 2, "byte": 3}
 MODE_AUTO, MODE_LEVELS, MODE_PERSISTENT = 0, 1, 2
 START_FEN = "rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1"
@@ -69,7 +69,7 @@ _lib = None
 # every symbol the C ABI declares (checked by the CPU test suite)
 EXPORTS = ["pf_init", "pf_destroy", "pf_strerror", "pf_fen_to_pos", "pf_pos_to_fen", "pf_perft",
            "pf_perft_batch", "pf_perft_batch_mode", "pf_divide", "pf_move_to_uci", "pf_apply_uci",
-           "pf_set_backend", "pf_set_dedup", "pf_tune", "pf_records", "pf_get_stats", "pf_reset_stats",
+           "pf_set_backend", "pf_set_dedup", "pf_tune", "pf_records", "pf_tt_save", "pf_tt_load", "pf_get_stats", "pf_reset_stats",
            "pf_sync", "pf_probe", "pf_playouts", "pf_count_moves_host", "pf_serve_start", "pf_submit", "pf_poll", "pf_serve_stop", "pf_now"]
 
 
@@ -109,6 +109,8 @@ def load_library(path=LIB_PATH):
         "pf_set_dedup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
         "pf_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
         "pf_records": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, u64p, u64p]),
+        "pf_tt_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+        "pf_tt_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
         "pf_submit": (ctypes.c_int, [ctypes.c_void_p, P(Pos48), ctypes.c_int, ctypes.c_uint64]),
         "pf_poll": (ctypes.c_int, [ctypes.c_void_p, P(Result), ctypes.c_int]),
         "pf_serve_start": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64]),
@@ -289,6 +291,14 @@ class Engine:
         _check(self._lib.pf_records(self._ctx, path.encode(), depth, batch, ctypes.byref(total), ctypes.byref(done)),
                "records(%r)" % path)
         return total.value, done.value
+
+    def tt_save(self, path):
+        """Write the transposition table to `path` (pf_tt_save)."""
+        _check(self._lib.pf_tt_save(self._ctx, path.encode()), "tt_save(%r)" % path)
+
+    def tt_load(self, path):
+        """Replace the transposition table with a saved one (pf_tt_load)."""
+        _check(self._lib.pf_tt_load(self._ctx, path.encode()), "tt_load(%r)" % path)
 
     def set_dedup(self, on):
         _check(self._lib.pf_set_dedup(self._ctx, 1 if on else 0), "set_dedup")

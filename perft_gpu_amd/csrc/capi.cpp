@@ -62,6 +62,7 @@ const char* pf_strerror(int s) {
     case PF_EBUSY: return "busy";
     case PF_EAGAIN: return "no result yet";
     case PF_EUNAVAIL: return "cooperative launch unavailable";
+    case PF_EIO: return "file input/output error";
     default: return "unknown status";
     }
 }
@@ -325,6 +326,20 @@ pf_status pf_set_dedup(pf_ctx* ctx, int on) {
     return PF_OK;
 }
 
+pf_status pf_tt_save(pf_ctx* ctx, const char* path) {
+    if (!ctx || !path) return PF_EINVAL;
+    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    if (ctx->eng.serving) return PF_EBUSY;
+    return ctx->eng.tt_save(path);
+}
+
+pf_status pf_tt_load(pf_ctx* ctx, const char* path) {
+    if (!ctx || !path) return PF_EINVAL;
+    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    if (ctx->eng.serving) return PF_EBUSY;
+    return ctx->eng.tt_load(path);
+}
+
 pf_status pf_get_stats(pf_ctx* ctx, pf_stats* out) {
     if (!ctx || !out) return PF_EINVAL;
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
@@ -401,6 +416,12 @@ pf_status pf_reset_stats(pf_ctx* ctx) {
     if (!ctx) return PF_EINVAL;
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
     ctx->eng.st = pf_stats{};
+    // the table counters accumulate on the device
+    if (ctx->eng.d_ttstats &&
+        (hipSetDevice(ctx->eng.device) != hipSuccess ||
+         hipMemsetAsync(ctx->eng.d_ttstats, 0, 8 * sizeof(unsigned long long), ctx->eng.stream) != hipSuccess ||
+         hipStreamSynchronize(ctx->eng.stream) != hipSuccess))
+        return PF_EDEVICE;
     return PF_OK;
 }
 

@@ -1,10 +1,12 @@
 // cli.cpp -- perft_gpu command line and UCI-style front end.
 //
-//   perft_gpu <fen|startpos> <depth> [backend] [tt_mib]
+//   perft_gpu <fen|startpos> <depth> [backend] [tt_mib] [table_file]
 //       the reference's CLI (perft.cu:29-137): runs perft for depths 1..depth
 //       and prints "GPU Perft d: N, Time taken: s, nps" per depth
 //       (launcher.h:1496-1498).  argv[3] selects the slider backend
-//       (ks|plain|fancy|byte) instead of the reference's GPU count.
+//       (ks|plain|fancy|byte) instead of the reference's GPU count; with a
+//       table, argv[5] names a file the table is restored from (if present)
+//       and saved to afterwards.
 //   perft_gpu uci
 //       UCI-style loop: uci, isready, ucinewgame, position startpos|fen F
 //       [moves ...], go perft N (divided perft + "Nodes searched: N"),
@@ -52,6 +54,16 @@ static int run_cli(int argc, char** argv) {
     pf_ctx* ctx = nullptr;
     int s = pf_init(&cfg, &ctx);
     if (s != PF_OK) { fprintf(stderr, "pf_init: %s\n", pf_strerror(s)); return 1; }
+    // argv[5]: table file, restored before the run when it exists and saved
+    // after it (the reference's perfthash.dat disk hash, launcher.h:700-789)
+    const char* ttfile = argc > 5 && cfg.tt_bytes ? argv[5] : nullptr;
+    if (ttfile) {
+        if (FILE* f = fopen(ttfile, "rb")) {
+            fclose(f);
+            s = pf_tt_load(ctx, ttfile);
+            if (s != PF_OK) { fprintf(stderr, "table %s: %s\n", ttfile, pf_strerror(s)); pf_destroy(ctx); return 1; }
+        }
+    }
     for (int d = 1; d <= depth; d++) {
         uint64_t n = 0;
         const auto t0 = std::chrono::steady_clock::now();
@@ -61,6 +73,11 @@ static int run_cli(int argc, char** argv) {
         printf("GPU Perft %d: %llu,   Time taken: %g seconds, nps: %llu\n", d, (unsigned long long)n, el,
                (unsigned long long)(el > 0 ? n / el : 0));
         fflush(stdout);
+    }
+    if (ttfile && (s = pf_tt_save(ctx, ttfile)) != PF_OK) {
+        fprintf(stderr, "table %s: %s\n", ttfile, pf_strerror(s));
+        pf_destroy(ctx);
+        return 1;
     }
     pf_destroy(ctx);
     return 0;

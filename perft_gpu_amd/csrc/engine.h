@@ -107,6 +107,8 @@ struct Engine {
   public:
     pf_status perft_levels_tt(const pf_pos48* pos, int n, int depth, u64* out);
     pf_status tt_clear();
+    pf_status tt_save(const char* path);
+    pf_status tt_load(const char* path);
 };
 
 }  // namespace pf

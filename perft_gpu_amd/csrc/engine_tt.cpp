@@ -228,8 +228,8 @@ pf_status Engine::perft_levels_tt(const pf_pos48* pos, int n, int d, u64* out) {
         hipMemcpyAsync(r.k2, h2.data(), (size_t)n * 8, hipMemcpyHostToDevice, stream) != hipSuccess ||
         hipMemsetAsync(d_total, 0, 8, stream) != hipSuccess)
         return fail(PF_EDEVICE);
-    if (launch_roots_tt(backend, d_tab, r.b, r.k1, r.k2, (u32)n, d, d_tt, tt_buckets - 1, r.acc, d_total, stream) !=
-        hipSuccess)
+    if (launch_roots_tt(backend, d_tab, r.b, r.k1, r.k2, (u32)n, d, d_tt, tt_buckets - 1, r.acc, d_total, d_ttstats,
+                        stream) != hipSuccess)
         return fail(PF_EDEVICE);
     st.kernel_launches++;
     if (hipMemcpyAsync(h_pinned + 4, d_total, 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||

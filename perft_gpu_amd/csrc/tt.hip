@@ -327,16 +327,21 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(kTTWaves))
 template <int B>
 __global__ __launch_bounds__(256) void k_roots_tt(const SliderTables* tabs, Board* roots, const u64* k1,
                                                   const u64* k2, u32 n, int depth, const uint4* tt, u64 mask,
-                                                  unsigned long long* acc, unsigned long long* total) {
+                                                  unsigned long long* acc, unsigned long long* total,
+                                                  unsigned long long* stats) {
     const auto sl = SlOfT<B>::make(tabs);
     const u32 i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const Board b = roots[i];
     u64 v = 0;
-    if (tt && depth >= kMinTTDepth && tt_probe(tt, mask, k1[i], k2[i], depth, &v)) {
-        acc[i] = v;
-        roots[i] = with_count(b, 0);  // answered: nothing to expand
-        return;
+    if (tt && depth >= kMinTTDepth) {
+        atomicAdd(&stats[0], 1ULL);
+        if (tt_probe(tt, mask, k1[i], k2[i], depth, &v)) {
+            atomicAdd(&stats[1], 1ULL);
+            acc[i] = v;
+            roots[i] = with_count(b, 0);  // answered: nothing to expand
+            return;
+        }
     }
     const u32 c = count_moves(b, sl);
     roots[i] = with_count(b, c);
@@ -524,13 +529,19 @@ hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables
 
 hipError_t launch_roots_tt(int backend, const SliderTables* tabs, Board* roots, const u64* k1, const u64* k2, u32 n,
                            int depth, const uint4* tt, u64 mask, unsigned long long* acc, unsigned long long* total,
-                           hipStream_t s) {
+                           unsigned long long* stats, hipStream_t s) {
     if (!n) return hipSuccess;
     const dim3 g((n + 255) / 256), b(256);
     switch (backend) {
-    case kKS: hipLaunchKernelGGL(k_roots_tt<kKS>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total); break;
-    case kPlain: hipLaunchKernelGGL(k_roots_tt<kPlain>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total); break;
-    default: hipLaunchKernelGGL(k_roots_tt<kFancy>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total); break;
+    case kKS:
+        hipLaunchKernelGGL(k_roots_tt<kKS>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total, stats);
+        break;
+    case kPlain:
+        hipLaunchKernelGGL(k_roots_tt<kPlain>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total, stats);
+        break;
+    default:
+        hipLaunchKernelGGL(k_roots_tt<kFancy>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total, stats);
+        break;
     }
     return hipGetLastError();
 }
@@ -540,6 +551,66 @@ hipError_t launch_upsweep(const UpSeg& u, uint4* tt, u64 mask, unsigned long lon
     const u32 items = u.n + u.ndup;
     if (!items) return hipSuccess;
     hipLaunchKernelGGL(k_upsweep, dim3((items + 255) / 256), dim3(256), 0, s, u, tt, mask, stores);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ int entry_depth(const uint4& e) { return (int)(e.w >> 24); }  // w1 >> 56
+
+// Folds entry e into a bucket held in registers, keeping its two deepest
+// entries (empty entries have depth 0).
+__device__ __forceinline__ void fold_entry(uint4& t0, uint4& t1, const uint4& e) {
+    if (!(e.x | e.y | e.z | e.w)) return;
+    const int d = entry_depth(e);
+    if (d >= entry_depth(t0)) {
+        t1 = t0;
+        t0 = e;
+    } else if (d >= entry_depth(t1)) {
+        t1 = e;
+    }
+}
+
+// Saved buckets [first, first + nb) of a table of src_buckets buckets -> this
+// table of mask + 1 buckets (both powers of two; this table is cleared
+// before the first chunk).  A bucket's index is the low bits of its
+// entries' first key:
+//  * same size or larger: source bucket s may hold keys of any target bucket
+//    s + k * src_buckets, so it is copied to each of them (the entries are
+//    still verified by their second key);
+//  * smaller: source buckets s, s + N, s + 2N, ... (N = mask + 1) fold into
+//    target s & mask.  One thread owns one target and folds all of the
+//    chunk's sources of it, so no two threads write one bucket.
+// The host keeps first + nb <= src_buckets.
+__global__ __launch_bounds__(256) void k_tt_refit(const uint4* src, u64 first, u64 nb, u64 src_buckets, uint4* tt,
+                                                  u64 mask) {
+    const u64 n = mask + 1, stride = (u64)gridDim.x * 256;
+    u64 i = (u64)blockIdx.x * 256 + threadIdx.x;
+    if (n >= src_buckets) {
+        const u64 total = nb * (n / src_buckets);
+        for (; i < total; i += stride) {
+            const u64 j = i % nb, k = i / nb;
+            uint4* t = tt + 2 * (first + j + k * src_buckets);
+            t[0] = src[2 * j];
+            t[1] = src[2 * j + 1];
+        }
+        return;
+    }
+    const u64 owners = nb < n ? nb : n;
+    for (; i < owners; i += stride) {
+        uint4* t = tt + 2 * ((first + i) & mask);
+        uint4 t0 = t[0], t1 = t[1];
+        for (u64 j = i; j < nb; j += n) {
+            fold_entry(t0, t1, src[2 * j]);
+            fold_entry(t0, t1, src[2 * j + 1]);
+        }
+        t[0] = t0;
+        t[1] = t1;
+    }
+}
+
+hipError_t launch_tt_refit(const uint4* src, u64 first, u64 nb, u64 src_buckets, uint4* tt, u64 mask, int n_cu,
+                           hipStream_t s) {
+    if (!nb) return hipSuccess;
+    hipLaunchKernelGGL(k_tt_refit, dim3(n_cu * 8), dim3(256), 0, s, src, first, nb, src_buckets, tt, mask);
     return hipGetLastError();
 }
 

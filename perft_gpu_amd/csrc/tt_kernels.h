@@ -140,11 +140,16 @@ hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt,
 int expand_tt_max_bpc(int backend);  // most resident workgroups per CU of the TT expand kernel
 hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables* zt, const TTArgs& a, int grid,
                             hipStream_t s);
+// Roots: probe (counted in stats[0], hits in stats[1]) or count their moves.
 hipError_t launch_roots_tt(int backend, const SliderTables* tabs, Board* roots, const u64* k1, const u64* k2, u32 n,
                            int depth, const uint4* tt, u64 mask, unsigned long long* acc, unsigned long long* total,
-                           hipStream_t s);
+                           unsigned long long* stats, hipStream_t s);
 // A finished level: its n nodes and ndup duplicates add their counts to their
 // parents (parent_acc null: the roots); nodes are stored in the table.
 hipError_t launch_upsweep(const UpSeg& u, uint4* tt, u64 mask, unsigned long long* stores, hipStream_t s);
+// A chunk of saved buckets [first, first + nb) of a src_buckets-bucket table
+// into this (cleared) table of mask + 1 buckets.
+hipError_t launch_tt_refit(const uint4* src, u64 first, u64 nb, u64 src_buckets, uint4* tt, u64 mask, int n_cu,
+                           hipStream_t s);
 
 }  // namespace pf

@@ -28,6 +28,7 @@ def build_harness():
         return HARNESS
     os.makedirs(os.path.dirname(HARNESS), exist_ok=True)
     subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
+                    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                     "-x", "hip"] + srcs + ["-o", HARNESS], check=True, stdout=subprocess.DEVNULL,
                    stderr=subprocess.DEVNULL)
     return HARNESS
