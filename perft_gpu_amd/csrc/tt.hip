@@ -74,34 +74,46 @@ __device__ __forceinline__ void tt_store(uint4* tt, u64 mask, u64 k1, u64 k2, in
 // range reduction of the high half of the first key.
 __device__ __forceinline__ u64 dup_slot(u64 k1, u64 n) { return ((k1 >> 32) * n) >> 32; }
 
-// A made child claims the first empty slot of its probe sequence.  Returns 0
-// when it claimed *slot (the first child of its position in this level: it
-// is written and publishes its index there), 1 when an earlier child with
-// its keys holds *slot (a duplicate), 2 when the probe sequence is full or
-// the key is 0 (written, no duplicate detection).
+// Outcomes of a made child in a write level (bits 0-1 of its code byte; bits
+// 2-6 hold the probe distance of its duplicate-table slot).
+enum ChildCode : u32 { kClaimed = 0, kDup = 1, kKeepNoSlot = 2, kSkip = 3 };
+
+__device__ __forceinline__ u64 probe_slot(u64 k1, u64 n, u32 dist) {
+    const u64 s = dup_slot(k1, n) + dist;
+    return s >= n ? s - n : s;
+}
+
+// A made child claims the first empty slot of its probe sequence.  Returns
+// kClaimed when it claimed slot probe_slot(k1, n, *dist) (the first child of
+// its position in this level: it is written and publishes its index there),
+// kDup when an earlier child with its keys holds that slot, kKeepNoSlot when
+// the probe sequence is full or the key is 0 (written, no duplicate
+// detection).
 // A slot goes from 0 to a key once in a level's life, so a non-zero key read
 // at device scope is final: a duplicate (most children at deep levels) is
 // recognised by a load, and only an empty slot needs the compare-and-swap.
-__device__ __forceinline__ int dup_claim(DupEntry* t, u64 n, u64 k1, u64 k2, u64* slot) {
-    if (!k2) return 2;
+__device__ __forceinline__ u32 dup_claim(DupEntry* t, u64 n, u64 k1, u64 k2, u32* dist) {
+    if (!k2) return kKeepNoSlot;
     u64 s = dup_slot(k1, n);
-    for (int p = 0; p < kDupProbes; p++) {
+    for (u32 p = 0; p < (u32)kDupProbes; p++) {
         unsigned long long cur = __hip_atomic_load(&t[s].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (cur == 0) {
             cur = atomicCAS(&t[s].key, 0ull, (unsigned long long)k2);
             if (cur == 0) {
-                *slot = s;
-                return 0;
+                *dist = p;
+                return kClaimed;
             }
         }
         if (cur == k2) {
-            *slot = s;
-            return 1;
+            *dist = p;
+            return kDup;
         }
         s = (s + 1 == n) ? 0 : s + 1;
     }
-    return 2;
+    return kKeepNoSlot;
 }
+
+static_assert(kDupProbes <= 32, "a probe distance fits in 5 bits of a child's code");
 
 struct TTLds {
     Board b[kTTP];
@@ -110,10 +122,15 @@ struct TTLds {
     u64 k1[kTTP], k2[kTTP];
     unsigned long long hit[kTTP];
     u32 mv[kTCAP];
+    unsigned char code[kTCAP];  // write levels: ChildCode | probe distance << 2 per window entry
     u32 wtot[kTWG / 64];
     unsigned long long red[kTWG / 64];
     u32 tile;
     u32 nprobe, nhit;
+    u32 nkeep, ndup;  // window totals from the claim pass
+    u32 kcur, dcur;   // window-local cursors of the write pass
+    u32 base, dbase;  // the window's ranges in the level and in the duplicate list
+    u32 ok;
 };
 
 struct LdsEmitT {
@@ -127,6 +144,150 @@ struct LdsEmitT {
     }
 };
 
+// The m children of one move-list window of a write level.  Every thread of
+// the block calls it; it ends with the window's writes issued.
+// Two passes, so that the level's and the duplicate list's cursors -- one
+// address each, shared by every workgroup, whose atomics serialise -- move
+// once per window rather than once per wave and 64 children:
+//  1. make, keys, table probe, duplicate claim: each child's outcome and
+//     probe distance go to s.code, the window's counts to s.nkeep / s.ndup;
+//  2. one thread reserves both ranges; every child to be written or listed
+//     as a duplicate is made again with its keys and placed at a window-local
+//     cursor inside the reserved range.
+template <class SL>
+__device__ __forceinline__ void write_window(const SL& sl, const TTArgs& a, const ZTables* zt, TTLds& s,
+                                             const u32 p0, const u32 m, const int lane, const int wid) {
+    const u64 below_mask = (1ULL << lane) - 1;
+    u32 wkeep = 0, wdup = 0, wprobe = 0, whit = 0;  // wave-uniform sums
+    for (u32 jb = (u32)wid * 64; jb < m; jb += kTWG) {
+        const u32 j = jb + lane;
+        u32 code = kSkip;
+        bool probed = false;
+        if (j < m) {
+            const u32 e = s.mv[j];
+            const u32 par = (e >> 16) & (kTTP - 1), mvv = e & 0xFFFFu;
+            const Board pb = s.b[par];
+            const Board c = make_move(pb, mvv);
+            u64 ck1 = 0, ck2 = 0;
+            if (a.tt) {
+                const int black = (int)(s.root[par] >> 31);
+                ck1 = zobrist_child(zt->k[0], s.k1[par], pb, mvv, black, c);
+                ck2 = zobrist_child(zt->k[1], s.k2[par], pb, mvv, black, c);
+            }
+            probed = a.tt && a.child_depth >= kMinTTDepth;
+            u64 v = 0;
+            if (probed && tt_probe(a.tt, a.tt_mask, ck1, ck2, a.child_depth, &v)) {
+                atomicAdd(&s.hit[par], (unsigned long long)v);
+            } else if (a.dup) {
+                u32 dist = 0;
+                code = dup_claim(a.dup, a.dup_n, ck1, ck2, &dist);
+                code |= dist << 2;
+            } else {
+                code = kKeepNoSlot;
+            }
+            s.code[j] = (unsigned char)code;
+        }
+        const u32 res = code & 3u;
+        wkeep += (u32)__popcll(__ballot(res == kClaimed || res == kKeepNoSlot));
+        wdup += (u32)__popcll(__ballot(res == kDup));
+        wprobe += (u32)__popcll(__ballot(probed));
+        whit += (u32)__popcll(__ballot(probed && res == kSkip));
+    }
+    if (lane == 0) {
+        if (wkeep) atomicAdd(&s.nkeep, wkeep);
+        if (wdup) atomicAdd(&s.ndup, wdup);
+        if (wprobe) {
+            atomicAdd(&s.nprobe, wprobe);
+            atomicAdd(&s.nhit, whit);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const u32 nk = s.nkeep, nd = s.ndup;
+        const u32 base = nk ? atomicAdd(a.out_count, nk) : 0u;
+        const u32 dbase = nd ? atomicAdd(a.dup_count, nd) : 0u;
+        const bool ok = (u64)base + nk <= a.out_cap && (u64)dbase + nd <= a.out_cap;
+        if (!ok) atomicOr(a.err, 1u);
+        s.base = base;
+        s.dbase = dbase;
+        s.ok = ok;
+        s.kcur = 0;
+        s.dcur = 0;
+        s.nkeep = 0;
+        s.ndup = 0;
+    }
+    __syncthreads();
+    if (!s.ok) return;  // uniform: the level overflowed its exact size
+    for (u32 jb = (u32)wid * 64; jb < m; jb += kTWG) {
+        const u32 j = jb + lane;
+        u32 code = kSkip, par = 0, mvv = 0;
+        if (j < m) {
+            code = s.code[j];
+            const u32 e = s.mv[j];
+            par = (e >> 16) & (kTTP - 1);
+            mvv = e & 0xFFFFu;
+        }
+        const u32 res = code & 3u;
+        const bool keep = res == kClaimed || res == kKeepNoSlot, isdup = res == kDup;
+        Board c = {};
+        u64 ck1 = 0, ck2 = 0;
+        u32 cc = 0;
+        if (keep || isdup) {
+            const Board pb = s.b[par];
+            c = make_move(pb, mvv);
+            if (a.tt) {
+                const int black = (int)(s.root[par] >> 31);
+                ck1 = zobrist_child(zt->k[0], s.k1[par], pb, mvv, black, c);
+                if (keep) ck2 = zobrist_child(zt->k[1], s.k2[par], pb, mvv, black, c);
+            }
+        }
+        if (keep) {
+            cc = count_moves(c, sl);
+            atomicAdd(&s.sum[par], cc);
+        }
+        const u64 kbal = __ballot(keep);
+        const u32 nk = (u32)__popcll(kbal);
+        u32 lb = 0;
+        if (lane == 0 && nk) lb = atomicAdd(&s.kcur, nk);
+        lb = __shfl(lb, 0, 64);
+        const u64 first = (u64)s.base + lb;  // the wave's written children are [first, first + nk)
+        const u64 pos = first + (u32)__popcll(kbal & below_mask);
+        if (keep) {
+            a.out[pos] = with_count(c, cc);
+            a.out_root[pos] = s.root[par] ^ 0x80000000u;  // child's side to move
+            if (a.tt) {
+                a.out_k1[pos] = ck1;
+                a.out_k2[pos] = ck2;
+            }
+            a.out_parent[pos] = a.in_base + p0 + par;
+            a.out_acc[pos] = 0;
+            if (res == kClaimed) a.dup[probe_slot(ck1, a.dup_n, code >> 2)].val = pos;  // publish the claimant
+        }
+        const u64 dbal = __ballot(isdup);
+        const u32 nd = (u32)__popcll(dbal);
+        if (nd) {
+            u32 dl = 0;
+            if (lane == 0) dl = atomicAdd(&s.dcur, nd);
+            dl = __shfl(dl, 0, 64);
+            if (isdup) {
+                const u64 dpos = (u64)s.dbase + dl + (u32)__popcll(dbal & below_mask);
+                a.dup_par[dpos] = a.in_base + p0 + par;
+                a.dup_slot[dpos] = (u32)probe_slot(ck1, a.dup_n, code >> 2);
+            }
+        }
+        if (a.out_bsum && nk) {
+            // the wave's children are consecutive: at most two blocks
+            const u64 b0 = first >> kBsumShift;
+            const u64 part = keep ? (((pos >> kBsumShift) == b0) ? (u64)cc : (u64)cc << 32) : 0;
+            const u64 t = t_wave_sum64(part);
+            if (lane == 0) {
+                if ((u32)t) atomicAdd(&a.out_bsum[b0], (u32)t);
+                if (t >> 32) atomicAdd(&a.out_bsum[b0 + 1], (u32)(t >> 32));
+            }
+        }
+    }
+}
+
 // One tile (kTTP parents) of one level.  Every thread of the block calls it.
 template <class SL>
 __device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTables* zt, TTLds& s) {
@@ -135,6 +296,8 @@ __device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTa
         if (tid == 0) {
             s.nprobe = 0;
             s.nhit = 0;
+            s.nkeep = 0;
+            s.ndup = 0;
         }
         const u32 p0 = tile * kTTP;
         const u32 np = min((u32)kTTP, a.n_in - p0);
@@ -174,105 +337,19 @@ __device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTa
             }
             __syncthreads();
             const u32 m = min((u32)kTCAP, tot - w0);
-            // each wave walks 64 consecutive entries at a time (uniform loop)
-            for (u32 jb = (u32)wid * 64; jb < m; jb += kTWG) {
-                const u32 j = jb + lane;
-                const bool valid = j < m;
-                u32 par = 0, mvv = 0;
-                if (valid) {
-                    const u32 e = s.mv[j];
-                    par = (e >> 16) & (kTTP - 1);
-                    mvv = e & 0xFFFFu;
-                }
-                Board c = {};
-                u32 cc = 0;
-                bool keep = false, probed = false, hit = false, isdup = false;
-                int claim = 2;
-                u64 ck1 = 0, ck2 = 0, dslot = 0;
-                if (valid) {
-                    const Board pb = s.b[par];
-                    c = make_move(pb, mvv);
-                    if (a.mode == kWrite) {
-                        u64 v = 0;
-                        if (a.tt) {
-                            const int black = (int)(s.root[par] >> 31);
-                            ck1 = zobrist_child(zt->k[0], s.k1[par], pb, mvv, black, c);
-                            ck2 = zobrist_child(zt->k[1], s.k2[par], pb, mvv, black, c);
-                        }
-                        probed = a.tt && a.child_depth >= kMinTTDepth;
-                        hit = probed && tt_probe(a.tt, a.tt_mask, ck1, ck2, a.child_depth, &v);
-                        if (hit) {
-                            atomicAdd(&s.hit[par], (unsigned long long)v);
-                        } else {
-                            if (a.dup) claim = dup_claim(a.dup, a.dup_n, ck1, ck2, &dslot);
-                            isdup = claim == 1;
-                            keep = !isdup;
-                        }
-                    }
-                    if (a.mode == kLeaf || keep) {
-                        cc = count_moves(c, sl);
+            if (a.mode == kLeaf) {
+                // each wave walks 64 consecutive entries at a time (uniform loop)
+                for (u32 jb = (u32)wid * 64; jb < m; jb += kTWG) {
+                    const u32 j = jb + lane;
+                    if (j < m) {
+                        const u32 e = s.mv[j];
+                        const u32 par = (e >> 16) & (kTTP - 1);
+                        const u32 cc = count_moves(make_move(s.b[par], e & 0xFFFFu), sl);
                         atomicAdd(&s.sum[par], cc);
                     }
                 }
-                if (a.mode == kWrite) {
-                    const u64 bal = __ballot(keep);
-                    const u32 nk = (u32)__popcll(bal);
-                    const u32 np_ = (u32)__popcll(__ballot(probed));
-                    const u32 nh = (u32)__popcll(__ballot(hit));
-                    if (lane == 0 && np_) {
-                        atomicAdd(&s.nprobe, np_);
-                        atomicAdd(&s.nhit, nh);
-                    }
-                    u32 base = 0;
-                    if (lane == 0 && nk) base = atomicAdd(a.out_count, nk);
-                    base = __shfl(base, 0, 64);
-                    const u32 below = (u32)__popcll(bal & ((1ULL << lane) - 1));
-                    const u64 pos = (u64)base + below;
-                    if (keep) {
-                        if (pos < a.out_cap) {
-                            a.out[pos] = with_count(c, cc);
-                            a.out_root[pos] = s.root[par] ^ 0x80000000u;  // child's side to move
-                            if (a.tt) {
-                                a.out_k1[pos] = ck1;
-                                a.out_k2[pos] = ck2;
-                            }
-                            a.out_parent[pos] = a.in_base + p0 + par;
-                            a.out_acc[pos] = 0;
-                            if (claim == 0) a.dup[dslot].val = pos;  // publish the claimant
-                        } else {
-                            atomicOr(a.err, 1u);
-                        }
-                    }
-                    if (a.dup) {
-                        const u64 dbal = __ballot(isdup);
-                        const u32 nd = (u32)__popcll(dbal);
-                        if (nd) {
-                            u32 dbase = 0;
-                            if (lane == 0) dbase = atomicAdd(a.dup_count, nd);
-                            dbase = __shfl(dbase, 0, 64);
-                            const u64 dpos = (u64)dbase + (u32)__popcll(dbal & ((1ULL << lane) - 1));
-                            if (isdup) {
-                                if (dpos < a.out_cap) {
-                                    a.dup_par[dpos] = a.in_base + p0 + par;
-                                    a.dup_slot[dpos] = (u32)dslot;
-                                } else {
-                                    atomicOr(a.err, 1u);
-                                }
-                            }
-                        }
-                    }
-                    if (a.out_bsum && nk) {
-                        // the wave's children are consecutive: at most two blocks
-                        const u64 b0 = (u64)base >> kBsumShift;
-                        const bool in = keep && pos < a.out_cap;
-                        const u64 part = in ? (((pos >> kBsumShift) == b0) ? (u64)cc : (u64)cc << 32) : 0;
-                        const u64 tot = t_wave_sum64(part);
-                        if (lane == 0) {
-                            if ((u32)tot) atomicAdd(&a.out_bsum[b0], (u32)tot);
-                            if (tot >> 32) atomicAdd(&a.out_bsum[b0 + 1], (u32)(tot >> 32));
-                        }
-                    }
-                }
+            } else {
+                write_window(sl, a, zt, s, p0, m, lane, wid);
             }
             __syncthreads();
         }
