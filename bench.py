@@ -215,6 +215,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--c5-positions", type=int, default=4096)
     ap.add_argument("--no-extras", action="store_true", help="skip every comparison run")
+    ap.add_argument("--sections", default="batch,serving,tt",
+                    help="comparison runs to do: batch (host buffers, plain, backends), serving, tt (C3, C5)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch shape")
     args = ap.parse_args()
@@ -296,7 +298,8 @@ def main():
                      "ops_per_leaf": W.C2_OPS_PER_LEAF},
     }
 
-    if not args.no_extras:
+    sections = set() if args.no_extras else set(s for s in args.sections.split(",") if s)
+    if "batch" in sections:
         # the same steps from host buffers: host-side conversion, 256 x 48-B
         # upload and the 2 KiB count read-back over PCIe on every step
         t = time.perf_counter()
@@ -332,7 +335,7 @@ def main():
         line["backends"] = be
     eng.close()
 
-    if not args.no_extras:
+    if "serving" in sections:
         try:
             line["serving"] = run_serving(pg, W, args.serve_seconds, args.backend)
             log("serving: p50 %.3f ms p99 %.3f ms" % (line["serving"]["p50_ms"], line["serving"]["p99_ms"]))
@@ -347,6 +350,7 @@ def main():
             log("serving capacity %.3e nodes/s vs plain %.3e" % (cap["nodes_per_s"], pl["nodes_per_s"]))
         except Exception as e:  # report, do not lose the headline
             line["serving_error"] = repr(e)
+    if "tt" in sections:
         try:
             import json as _j
             with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
