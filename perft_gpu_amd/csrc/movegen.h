@@ -40,13 +40,17 @@ struct PosInfo {
     int ksq;
 };
 
-PF_HD u64 between_on(u64 line, int a, int b) {
+PF_HD u64 between_on(u64 line, int a, int b) {  // squares of `line` strictly between a and b
     int lo = a < b ? a : b, hi = a < b ? b : a;
-    return line & ((bit(hi) - 1) ^ ((bit(lo) << 1) - 1));
+    return line & (bit(hi) - (bit(lo) << 1));
 }
 
-template <class SL>
-PF_HD void analyse(const Board& b, const SL& sl, PosInfo& pi) {
+// Everything of PosInfo but the threat map.  `xray` (when not null) receives,
+// for each sliding checker, the square just beyond the king on its line: the
+// one square that the threat map with our king removed (findAttackedSquares'
+// x-ray) adds to a map computed with the king in place, among the squares a
+// king move can reach.
+PF_HD void pins_checks(const Board& b, PosInfo& pi, u64* xray) {
     const u64 pawns = b.pawns & kRanks2to7;
     const u64 occ = pawns | b.knights | b.bishops | b.rooks | b.kings;
     const u64 us = b.us, them = occ & ~us;
@@ -54,13 +58,9 @@ PF_HD void analyse(const Board& b, const SL& sl, PosInfo& pi) {
     const int ksq = ctz(K);
     const u64 eB = b.bishops & them, eR = b.rooks & them;
     const u64 eP = pawns & them, eN = b.knights & them;
-    const u64 occX = occ ^ K;
-    u64 att = ((eP >> 7) & kNotA) | ((eP >> 9) & kNotH);
-    att |= knight_fill(eN) | king_fill(b.kings & them);
-    att |= sl.bishop_multi(eB, occX) | sl.rook_multi(eR, occX);
-
-    u64 pinned = 0, checkers = 0, block = 0;
+    u64 pinned = 0, checkers = 0, block = 0, beyond = 0;
     const u64 kf = file_of(ksq), kr = rank_of(ksq), kd = diag_of(ksq), ka = anti_of(ksq);
+    const u64 kz = xray ? king_fill(K) : 0;
     u64 sn = ((kf | kr) & eR) | ((kd | ka) & eB);
     while (sn) {
         const int s = ctz(sn);
@@ -69,12 +69,31 @@ PF_HD void analyse(const Board& b, const SL& sl, PosInfo& pi) {
         const u64 line = (kf & sb) ? kf : (kr & sb) ? kr : (kd & sb) ? kd : ka;
         const u64 btw = between_on(line, ksq, s);
         const u64 in = btw & occ;
-        if (!in) { checkers |= sb; block |= btw; }
-        else if (!(in & (in - 1))) pinned |= in & us;
+        if (!in) {
+            checkers |= sb;
+            block |= btw;
+            beyond |= line & kz & ~(btw | sb);  // the king's neighbour on the line, away from the checker
+        } else if (!(in & (in - 1))) {
+            pinned |= in & us;
+        }
     }
     checkers |= (knight_fill(K) & eN) | ((((K << 7) & kNotH) | ((K << 9) & kNotA)) & eP);
+    if (xray) *xray = beyond;
     pi.occ = occ; pi.us = us; pi.them = them; pi.pawns = pawns; pi.K = K;
-    pi.att = att; pi.pinned = pinned; pi.checkers = checkers; pi.block = block; pi.ksq = ksq;
+    pi.att = 0; pi.pinned = pinned; pi.checkers = checkers; pi.block = block; pi.ksq = ksq;
+}
+
+// Squares attacked by the opponent's pawns, knights and king.
+PF_HD u64 leaper_threats(const Board& b, const PosInfo& pi) {
+    const u64 eP = pi.pawns & pi.them;
+    return ((eP >> 7) & kNotA) | ((eP >> 9) & kNotH) | knight_fill(b.knights & pi.them) | king_fill(b.kings & pi.them);
+}
+
+template <class SL>
+PF_HD void analyse(const Board& b, const SL& sl, PosInfo& pi) {
+    pins_checks(b, pi, nullptr);
+    const u64 occX = pi.occ ^ pi.K;  // x-ray through our king (findAttackedSquares)
+    pi.att = leaper_threats(b, pi) | sl.bishop_multi(b.bishops & pi.them, occX) | sl.rook_multi(b.rooks & pi.them, occX);
 }
 
 // Knight moves of every knight in `n` onto `tgt`: the eight jump directions
@@ -127,20 +146,95 @@ PF_HD int slider_count(const SL& sl, u64 B, u64 R, u64 occ, u64 tgt, u64 pinned,
 // in double check `safe` is empty and only the king moves.  (A wave holds 64
 // different positions, so a separate in-check path would run for nearly
 // every wave.)
+struct CountMasks {
+    bool check;
+    u64 safe;    // destinations that resolve a check (every square out of check)
+    u64 tgt;     // non-king destinations: not ours, and safe
+    u64 pinMov;  // pinned pieces that move along their pin line (none in check)
+};
+PF_HD CountMasks count_masks(const PosInfo& pi) {
+    CountMasks m;
+    m.check = pi.checkers != 0;
+    const bool dbl = (pi.checkers & (pi.checkers - 1)) != 0;
+    m.safe = !m.check ? ~0ULL : dbl ? 0ULL : (pi.checkers | pi.block);
+    m.tgt = ~pi.us & m.safe;
+    m.pinMov = m.check ? 0ULL : pi.pinned;
+    return m;
+}
+
+// King moves and castling, given the threat map `att`.
+PF_HD int count_king(const Board& b, const PosInfo& pi, const CountMasks& cm, u64 att) {
+    int n = popc(king_fill(pi.K) & ~pi.us & ~att);
+    // castling (MoveGeneratorBitboard.h:3180-3198), never out of check
+    const int r = st_rights(b);
+    n += (!cm.check && (r & 1) && !(pi.occ & kF1G1) && !(att & kF1G1)) ? 1 : 0;
+    n += (!cm.check && (r & 2) && !(pi.occ & kB1D1) && !(att & kC1D1)) ? 1 : 0;
+    return n;
+}
+
+// Pawn, en-passant and knight moves.
+PF_HD int count_pawns_knights(const Board& b, const PosInfo& pi, const CountMasks& cm);
+
+// Number of legal moves of the Kogge-Stone backend: the threat map's slider
+// rays and our sliders' moves are filled direction by direction over one
+// shared propagator chain, the map with our king in place plus the squares
+// beyond it (pins_checks).
+PF_HD u32 count_moves_ks(const Board& b) {
+    PosInfo pi;
+    u64 att = 0;
+    pins_checks(b, pi, &att);
+    const CountMasks cm = count_masks(pi);
+    att |= leaper_threats(b, pi);
+    const u64 them = pi.them, us = pi.us, empty = ~pi.occ, tgt = cm.tgt;
+    const u64 eB = b.bishops & them, eR = b.rooks & them;
+    // our sliders: pinned ones only along their pin line, none while in check
+    const u64 frozen = pi.pinned & ~cm.pinMov, free = ~cm.pinMov;
+    const u64 oB = b.bishops & us & ~frozen, oR = b.rooks & us & ~frozen;
+    const int ksq = pi.ksq;
+    const u64 rv = oR & (free | file_of(ksq)), rh = oR & (free | rank_of(ksq));
+    const u64 bd = oB & (free | diag_of(ksq)), ba = oB & (free | anti_of(ksq));
+    int n = 0;
+#define PF_KS_DIR(D, E, O)                        \
+    {                                             \
+        const RayChain c = ray_chain<D>(empty);   \
+        att |= ray_fill<D>(E, c);                 \
+        n += popc(ray_fill<D>(O, c) & tgt);       \
+    }
+    PF_KS_DIR(8, eR, rv)
+    PF_KS_DIR(-8, eR, rv)
+    PF_KS_DIR(1, eR, rh)
+    PF_KS_DIR(-1, eR, rh)
+    PF_KS_DIR(9, eB, bd)
+    PF_KS_DIR(-9, eB, bd)
+    PF_KS_DIR(7, eB, ba)
+    PF_KS_DIR(-7, eB, ba)
+#undef PF_KS_DIR
+    return (u32)(n + count_king(b, pi, cm, att) + count_pawns_knights(b, pi, cm));
+}
+
 template <class SL>
 PF_HD u32 count_moves(const Board& b, const SL& sl) {
-    PosInfo pi;
-    analyse(b, sl, pi);
+    if constexpr (SL::kSetwise) {
+        return count_moves_ks(b);
+    } else {
+        PosInfo pi;
+        analyse(b, sl, pi);
+        const CountMasks cm = count_masks(pi);
+        const u64 frozen = pi.pinned & ~cm.pinMov;
+        return (u32)(count_king(b, pi, cm, pi.att) + count_pawns_knights(b, pi, cm) +
+                     slider_count(sl, b.bishops & pi.us & ~frozen, b.rooks & pi.us & ~frozen, pi.occ, cm.tgt,
+                                  cm.pinMov, pi.ksq));
+    }
+}
+
+PF_HD int count_pawns_knights(const Board& b, const PosInfo& pi, const CountMasks& cm) {
     const u64 occ = pi.occ, us = pi.us, them = pi.them, empty = ~occ;
     const u64 P = pi.pawns & us;
     const u64 Pf = P & ~pi.pinned;
     const int ksq = pi.ksq;
-    const bool check = pi.checkers != 0;
-    const bool dbl = (pi.checkers & (pi.checkers - 1)) != 0;
-    const u64 safe = !check ? ~0ULL : dbl ? 0ULL : (pi.checkers | pi.block);
-    const u64 tgt = ~us & safe;
-    const u64 pinMov = check ? 0ULL : pi.pinned;  // pinned pieces that move along their line
-    int n = popc(king_fill(pi.K) & ~us & ~pi.att);
+    const bool check = cm.check;
+    const u64 safe = cm.safe, pinMov = cm.pinMov;
+    int n = 0;
 
     // pawns (MoveGeneratorBitboard.h:3109-3178; in check :2892-2940)
     {
@@ -191,15 +285,9 @@ PF_HD u32 count_moves(const Board& b, const SL& sl) {
             }
         }
     }
-    // castling (MoveGeneratorBitboard.h:3180-3198), never out of check
-    const int r = st_rights(b);
-    n += (!check && (r & 1) && !(occ & kF1G1) && !(pi.att & kF1G1)) ? 1 : 0;
-    n += (!check && (r & 2) && !(occ & kB1D1) && !(pi.att & kC1D1)) ? 1 : 0;
-    // knights (pinned ones never move) and sliders (pinned ones along their line, never in check)
-    n += knight_count(b.knights & us & ~pi.pinned, tgt);
-    const u64 frozen = pi.pinned & ~pinMov;
-    n += slider_count(sl, b.bishops & us & ~frozen, b.rooks & us & ~frozen, occ, tgt, pinMov, ksq);
-    return (u32)n;
+    // knights (pinned ones never move)
+    n += knight_count(b.knights & us & ~pi.pinned, cm.tgt);
+    return n;
 }
 
 // Legal move generation.  `emit(move)` is called once per legal move; the
