@@ -293,7 +293,7 @@ def main():
                      "frac": achieved / VALU_PEAK_TOPS,
                      # HBM bytes per launch: FETCH_SIZE x2.00 + WRITE_SIZE x1.00 (rocprofv3 --pmc, calibrated
                      # on a copy of known size; profiles/k_persistent_ks_c2_d5_counters.md)
-                     "traffic": 9.139e8,
+                     "traffic": 9.133e8,
                      "kernel": "k_persistent<%s>" % args.backend, "kernel_ms": kernel_ms,
                      "ops_per_leaf": W.C2_OPS_PER_LEAF},
     }
