@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip every comparison run")
     ap.add_argument("--sections", default="batch,serving,tt",
                     help="comparison runs to do: batch (host buffers, plain, backends), serving, tt (C3, C5)")
+    ap.add_argument("--serve-modes", default="paced,capacity,plain",
+                    help="serving runs: paced (real-time arrivals), capacity (all at once), plain (per request)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch shape")
     args = ap.parse_args()
@@ -337,17 +339,25 @@ def main():
 
     if "serving" in sections:
         try:
-            line["serving"] = run_serving(pg, W, args.serve_seconds, args.backend)
-            log("serving: p50 %.3f ms p99 %.3f ms" % (line["serving"]["p50_ms"], line["serving"]["p99_ms"]))
-            cap = run_serving(pg, W, args.serve_seconds, args.backend, paced=False)
-            line["serving_capacity"] = cap
-            pl = run_plain_requests(pg, W, args.serve_seconds, args.backend)
-            sc, cc, pc = line["serving"].pop("_counts"), cap.pop("_counts"), pl.pop("_counts")
-            line["serving"]["counts_match_plain"] = sc == pc
-            cap["counts_match_plain"] = cc == pc
-            line["plain_same_requests"] = pl
-            line["served_over_plain"] = cap["nodes_per_s"] / pl["nodes_per_s"]
-            log("serving capacity %.3e nodes/s vs plain %.3e" % (cap["nodes_per_s"], pl["nodes_per_s"]))
+            modes = set(m for m in args.serve_modes.split(",") if m)
+            runs = {}
+            if "paced" in modes:
+                runs["serving"] = run_serving(pg, W, args.serve_seconds, args.backend)
+                log("serving: p50 %.3f ms p99 %.3f ms" % (runs["serving"]["p50_ms"], runs["serving"]["p99_ms"]))
+            if "capacity" in modes:
+                runs["serving_capacity"] = run_serving(pg, W, args.serve_seconds, args.backend, paced=False)
+            if "plain" in modes:
+                runs["plain_same_requests"] = run_plain_requests(pg, W, args.serve_seconds, args.backend)
+            counts = {k: r.pop("_counts") for k, r in runs.items()}
+            if "plain_same_requests" in runs:
+                for k in ("serving", "serving_capacity"):
+                    if k in runs:
+                        runs[k]["counts_match_plain"] = counts[k] == counts["plain_same_requests"]
+                if "serving_capacity" in runs:
+                    cap, pl = runs["serving_capacity"], runs["plain_same_requests"]
+                    line["served_over_plain"] = cap["nodes_per_s"] / pl["nodes_per_s"]
+                    log("serving capacity %.3e nodes/s vs plain %.3e" % (cap["nodes_per_s"], pl["nodes_per_s"]))
+            line.update(runs)
         except Exception as e:  # report, do not lose the headline
             line["serving_error"] = repr(e)
     if "tt" in sections:
