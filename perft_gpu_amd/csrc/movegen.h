@@ -194,21 +194,45 @@ PF_HD u32 count_moves_ks(const Board& b) {
     const u64 rv = oR & (free | file_of(ksq)), rh = oR & (free | rank_of(ksq));
     const u64 bd = oB & (free | diag_of(ksq)), ba = oB & (free | anti_of(ksq));
     int n = 0;
-#define PF_KS_DIR(D, E, O)                        \
-    {                                             \
-        const RayChain c = ray_chain<D>(empty);   \
-        att |= ray_fill<D>(E, c);                 \
-        n += popc(ray_fill<D>(O, c) & tgt);       \
+    // the final step's wrap mask folds into the destination masks
+    const u64 tgtA = tgt & kNotA, tgtH = tgt & kNotH;
+    u64 attA = 0, attH = 0;  // threat rays that still need the kNotA / kNotH mask
+#define PF_KS_DIR(D, C, E, O, ATT, TGT)                 \
+    {                                                   \
+        ATT |= step<D>(ray_reach<D>(E, C));             \
+        n += popc(step<D>(ray_reach<D>(O, C)) & TGT);   \
     }
-    PF_KS_DIR(8, eR, rv)
-    PF_KS_DIR(-8, eR, rv)
-    PF_KS_DIR(1, eR, rh)
-    PF_KS_DIR(-1, eR, rh)
-    PF_KS_DIR(9, eB, bd)
-    PF_KS_DIR(-9, eB, bd)
-    PF_KS_DIR(7, eB, ba)
-    PF_KS_DIR(-7, eB, ba)
+    {
+        const RayChain s = ray_chain<-8>(empty), nn = north_chain(s);
+        PF_KS_DIR(8, nn, eR, rv, att, tgt)
+        PF_KS_DIR(-8, s, eR, rv, att, tgt)
+    }
+    {
+        const RayChain c = ray_chain<1>(empty);
+        PF_KS_DIR(1, c, eR, rh, attA, tgtA)
+    }
+    {
+        const RayChain c = ray_chain<-1>(empty);
+        PF_KS_DIR(-1, c, eR, rh, attH, tgtH)
+    }
+    {
+        const RayChain c = ray_chain<9>(empty);
+        PF_KS_DIR(9, c, eB, bd, attA, tgtA)
+    }
+    {
+        const RayChain c = ray_chain<-9>(empty);
+        PF_KS_DIR(-9, c, eB, bd, attH, tgtH)
+    }
+    {
+        const RayChain c = ray_chain<7>(empty);
+        PF_KS_DIR(7, c, eB, ba, attH, tgtH)
+    }
+    {
+        const RayChain c = ray_chain<-7>(empty);
+        PF_KS_DIR(-7, c, eB, ba, attA, tgtA)
+    }
 #undef PF_KS_DIR
+    att = or_and(or_and(att, attA, kNotA), attH, kNotH);
     return (u32)(n + count_king(b, pi, cm, att) + count_pawns_knights(b, pi, cm));
 }
 
@@ -238,18 +262,18 @@ PF_HD int count_pawns_knights(const Board& b, const PosInfo& pi, const CountMask
 
     // pawns (MoveGeneratorBitboard.h:3109-3178; in check :2892-2940)
     {
-        const u64 p1 = (Pf << 8) & empty;
+        // Pinned pawns move only along their pin line: file-pinned ones push,
+        // a1-h8-pinned ones capture up-right, a8-h1-pinned ones up-left.
+        // pinMov is empty in check, and out of check `safe` is every square,
+        // so they share the unpinned pawns' masks.
+        const u64 push = Pf | (P & pinMov & file_of(ksq));
+        const u64 capR = Pf | (P & pinMov & diag_of(ksq)), capL = Pf | (P & pinMov & anti_of(ksq));
+        const u64 p1 = (push << 8) & empty;
         n += popc(p1 & safe) + 3 * popc(p1 & safe & kRank8);
         const u64 dmask = check ? ~safe : ~0ULL;
         n += popc((((p1 & dmask) & kRank3) << 8) & empty & safe);
-        const u64 cl = ((Pf << 7) & kNotH) & them & safe, cr = ((Pf << 9) & kNotA) & them & safe;
+        const u64 cl = ((capL << 7) & kNotH) & them & safe, cr = ((capR << 9) & kNotA) & them & safe;
         n += popc(cl) + popc(cr) + 3 * (popc(cl & kRank8) + popc(cr & kRank8));
-        const u64 pinF = pinMov & file_of(ksq), pinD = pinMov & diag_of(ksq), pinA = pinMov & anti_of(ksq);
-        const u64 q1 = ((P & pinF) << 8) & empty;  // file-pinned pawns push along the pin
-        n += popc(q1) + popc(((q1 & kRank3) << 8) & empty);
-        const u64 qd = (((P & pinD) << 9) & kNotA) & them;  // stays on the a1-h8 pin line
-        const u64 qa = (((P & pinA) << 7) & kNotH) & them;  // stays on the a8-h1 pin line
-        n += popc(qd) + popc(qa) + 3 * (popc(qd & kRank8) + popc(qa & kRank8));
     }
     // en passant (MoveGeneratorBitboard.h:3058-3107; in check :2873-2890)
     const int ep = st_ep(b);
