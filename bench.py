@@ -20,8 +20,9 @@ Also reported on the same box, outside the timed region:
                 32 GiB shared TT: p50/p99 latency and nodes/sec
   c3            one 20-30-ply middlegame position, perft 8, 64 GiB TT with
                 in-level duplicate elimination (and perft 7 against the
-                fixture, and perft 8 with the TT off); HBM random-access
-                roofline from the live random-access and copy probes
+                fixture, and perft 8 with the TT off); two-phase roofline:
+                table traffic at the live random-access and copy rates plus
+                the counting work at the vector peak (tt_roofline)
   c5            the 4096 high-mobility positions (--c5-positions), perft 6,
                 128 GiB TT
   cpu_baseline  the oracle (CPU restatement of the reference perft, fancy
@@ -172,16 +173,30 @@ FRONTIER_NODE_BYTES = 80  # board 48 + root word 4 + two keys 16 + parent 4 + ac
 
 
 def tt_roofline(st, seconds, ceil):
-    """HBM random-access roofline of a TT run: every probe, store, duplicate-
-    slot claim and duplicate-slot lookup is one random access of at most
-    32 bytes, priced at the measured random 16-byte rate; every frontier node
-    is written once and read once at streaming rate.  t_min / t_run."""
-    rnd = st["tt_probes"] + st["tt_stores"] + 2 * st["frontier_nodes"]
-    stream = 2 * FRONTIER_NODE_BYTES * st["frontier_nodes"]
-    t_min = rnd / ceil["random16_per_s"] + stream / ceil["copy_bytes_per_s"]
-    return {"bound": "hbm-random", "random_accesses": rnd, "stream_bytes": stream, "t_min_s": t_min,
-            "random16_per_s": ceil["random16_per_s"], "copy_bytes_per_s": ceil["copy_bytes_per_s"],
-            "frac": t_min / seconds}
+    """Roofline of a TT run as two phases that run one after the other: the
+    table and duplicate traffic (HBM random access) and the move counting
+    (integer VALU).
+      memory: every probe, store and duplicate claim, and the duplicates'
+              two reads in the upsweep, is one random access of at most
+              32 bytes, priced at the measured random 16-byte rate; every
+              frontier node is written once and read once at streaming rate;
+      valu:   every made child that is counted (not a table hit, not a
+              duplicate) costs the C2 op model's lane-ops per made node
+              (make + count, and generate where it is expanded further),
+              priced at the published vector peak.
+    frac = (t_memory + t_valu) / t_run."""
+    from perft_gpu_amd.workloads import C2_OPS_PER_MADE_NODE as W_OPS_PER_MADE_NODE
+    dups, front = st["dup_nodes"], st["frontier_nodes"]
+    rnd = st["tt_probes"] + st["tt_stores"] + front + 3 * dups
+    stream = 2 * FRONTIER_NODE_BYTES * front
+    t_mem = rnd / ceil["random16_per_s"] + stream / ceil["copy_bytes_per_s"]
+    counted = max(0, st["made_nodes"] - dups - st["tt_hits"])
+    t_valu = counted * W_OPS_PER_MADE_NODE / (VALU_PEAK_TOPS * 1e12)
+    return {"bound": "hbm-random+valu", "random_accesses": rnd, "stream_bytes": stream, "t_memory_s": t_mem,
+            "counted_nodes": counted, "ops_per_counted_node": W_OPS_PER_MADE_NODE, "t_valu_s": t_valu,
+            "t_min_s": t_mem + t_valu, "random16_per_s": ceil["random16_per_s"],
+            "copy_bytes_per_s": ceil["copy_bytes_per_s"], "memory_frac": t_mem / seconds,
+            "frac": (t_mem + t_valu) / seconds}
 
 
 def run_tt_workload(pg, W, cfg, positions, depth, backend, arena, check=None, ceil=None, tt=True):
@@ -195,7 +210,7 @@ def run_tt_workload(pg, W, cfg, positions, depth, backend, arena, check=None, ce
         out = {"depth": depth, "positions": len(positions), "nodes": int(got.sum()), "seconds": el,
                "nodes_per_s": float(got.sum()) / el, "tt_gib": (cfg["tt_bytes"] >> 30) if tt else 0,
                "tt_probes": st["tt_probes"], "tt_hits": st["tt_hits"], "tt_stores": st["tt_stores"],
-               "dup_nodes": st["dup_nodes"], "frontier_nodes": st["frontier_nodes"],
+               "dup_nodes": st["dup_nodes"], "frontier_nodes": st["frontier_nodes"], "made_nodes": st["made_nodes"],
                "chunks": st["chunks"], "peak_arena_gib": st["peak_arena_bytes"] / 2 ** 30}
         if tt and ceil:
             out["roofline"] = tt_roofline(st, el, ceil)

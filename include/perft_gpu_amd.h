@@ -94,6 +94,8 @@ typedef struct pf_stats {
     uint64_t last_expand_launches;
     uint64_t dup_nodes;       /* nodes found to duplicate another node of their level (TT runs) */
     uint64_t frontier_nodes;  /* nodes written to frontier levels (TT runs) */
+    uint64_t made_nodes;      /* children made and counted by the level kernels (TT runs): hits,
+                                 duplicates, written nodes and the leaf pass's children */
 } pf_stats;
 
 /* Replaces initGPU + MoveGeneratorBitboard::init + setupHashTables128b

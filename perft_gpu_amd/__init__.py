@@ -58,7 +58,7 @@ class Stats(ctypes.Structure):
                 ("peak_arena_bytes", ctypes.c_uint64), ("tt_probes", ctypes.c_uint64), ("tt_hits", ctypes.c_uint64),
                 ("tt_stores", ctypes.c_uint64), ("gpu_ms", ctypes.c_double), ("last_expand_ms", ctypes.c_double),
                 ("last_expand_launches", ctypes.c_uint64), ("dup_nodes", ctypes.c_uint64),
-                ("frontier_nodes", ctypes.c_uint64)]
+                ("frontier_nodes", ctypes.c_uint64), ("made_nodes", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -145,7 +145,10 @@ static u64 dup_slots(bool on, u64 n, int ply) {
 // Parents [off, off + m) of level `lv` (level L) have n_next children in all.
 pf_status Engine::descend_tt(const TLevel& lv, u64 off, u64 m, u64 n_next, int L) {
     if (!m) return PF_OK;
-    if (depth - L == 2) return expand_tt(kLeaf, lv, off, m, nullptr, 0, L, nullptr, nullptr);
+    if (depth - L == 2) {
+        st.made_nodes += n_next;
+        return expand_tt(kLeaf, lv, off, m, nullptr, 0, L, nullptr, nullptr);
+    }
     // the children's level, and for ply >= 3 their duplicate table and list,
     // live until the level's upsweep
     const u64 dn = dup_slots(dedup, n_next, L + 1);
@@ -165,6 +168,7 @@ pf_status Engine::descend_tt(const TLevel& lv, u64 off, u64 m, u64 n_next, int L
         pf_status s = PF_OK;
         if (nx.dup && hipMemsetAsync(nx.dup, 0, dn * sizeof(DupEntry), stream) != hipSuccess) s = PF_EDEVICE;
         u64 written = 0, n_next2 = 0;
+        st.made_nodes += n_next;
         if (s == PF_OK) s = expand_tt(kWrite, lv, off, m, &nx, n_next, L, &written, &n_next2);
         if (s == PF_OK) s = descend_tt(nx, 0, written, n_next2, L + 1);
         if (s == PF_OK && written + nx.ndup) {

@@ -91,3 +91,9 @@ def request_position(pos_seed, cfg=C4):
 # (tests/golden/make_golden.py): 416,777,397,783 ops / 8,988,346,871 leaves.
 C2_OPS_PER_LEAF = 416777397783 / 8988346871
 C2_LEAVES = 8988346871
+# The same op total per node made below the root (plies 1-4 of the C2 tree:
+# 8,001 + 249,956 + 8,164,767 + 265,225,172 = 273,647,896 nodes, each made and
+# counted, the interior ones also generated): the lane-op cost of one made
+# and counted node, used to price the counting work of table runs.
+C2_MADE_NODES = 8001 + 249956 + 8164767 + 265225172
+C2_OPS_PER_MADE_NODE = 416777397783 / C2_MADE_NODES

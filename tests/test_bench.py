@@ -1,0 +1,43 @@
+"""bench.py host logic on the CPU: the one-GPU guard and the table-run
+roofline arithmetic (no GPU is touched)."""
+import json
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_tt_roofline_phases_add_up():
+    sys.path.insert(0, ROOT)
+    import bench
+    from perft_gpu_amd.workloads import C2_OPS_PER_MADE_NODE
+    ceil = {"random16_per_s": 2e10, "copy_bytes_per_s": 5e12}
+    st = {"tt_probes": 1000, "tt_stores": 500, "tt_hits": 100, "frontier_nodes": 4000, "dup_nodes": 2000,
+          "made_nodes": 10000}
+    r = bench.tt_roofline(st, 1e-3, ceil)
+    rnd = 1000 + 500 + 4000 + 3 * 2000
+    t_mem = rnd / 2e10 + 2 * bench.FRONTIER_NODE_BYTES * 4000 / 5e12
+    t_valu = (10000 - 2000 - 100) * C2_OPS_PER_MADE_NODE / (bench.VALU_PEAK_TOPS * 1e12)
+    assert r["random_accesses"] == rnd and r["counted_nodes"] == 7900
+    assert abs(r["t_memory_s"] - t_mem) < 1e-15 and abs(r["t_valu_s"] - t_valu) < 1e-15
+    assert abs(r["frac"] - (t_mem + t_valu) / 1e-3) < 1e-12
+    json.dumps(r)
+
+
+def test_op_model_per_made_node_matches_the_fixture():
+    """the C2 tree's made nodes are its perft 1..4 sums, and the op total is
+    the fixture's"""
+    with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
+        c2 = json.load(f)["c2"]
+    from perft_gpu_amd import workloads as W
+    made = sum(sum(c2["perft"][str(d)]) for d in range(1, 5))
+    assert W.C2_MADE_NODES == made
+    assert abs(W.C2_OPS_PER_MADE_NODE - c2["op_model"]["int32_lane_ops"] / made) < 1e-9
