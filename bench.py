@@ -363,11 +363,11 @@ def main():
                 runs["serving_capacity"] = run_serving(pg, W, args.serve_seconds, args.backend, paced=False)
             if "plain" in modes:
                 runs["plain_same_requests"] = run_plain_requests(pg, W, args.serve_seconds, args.backend)
-            counts = {k: r.pop("_counts") for k, r in runs.items()}
+            req_counts = {k: r.pop("_counts") for k, r in runs.items()}
             if "plain_same_requests" in runs:
                 for k in ("serving", "serving_capacity"):
                     if k in runs:
-                        runs[k]["counts_match_plain"] = counts[k] == counts["plain_same_requests"]
+                        runs[k]["counts_match_plain"] = req_counts[k] == req_counts["plain_same_requests"]
                 if "serving_capacity" in runs:
                     cap, pl = runs["serving_capacity"], runs["plain_same_requests"]
                     line["served_over_plain"] = cap["nodes_per_s"] / pl["nodes_per_s"]
