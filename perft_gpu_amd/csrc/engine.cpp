@@ -144,8 +144,9 @@ pf_status Engine::expand(int mode, const Board* in, const u32* in_root, u64 n_in
     a.total = next_total ? d_total : nullptr;
     a.tile_ctr = d_ctrl + 1;
     a.err = d_ctrl + 0;
-    const u64 tiles = (n_in + 255) / 256;
     int g = grid[backend][mode];
+    a.tile_parents = level_tile_parents(n_in, (u64)g, 256);
+    const u64 tiles = (n_in + a.tile_parents - 1) / a.tile_parents;
     if ((u64)g > tiles) g = (int)tiles;
     if (timing) PF_HIP(hipEventRecord(ev0, stream));
     PF_HIP(launch_expand(backend, mode, DevTables{d_tab}, a, g, stream));

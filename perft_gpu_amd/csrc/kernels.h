@@ -25,7 +25,16 @@ struct LevelArgs {
     unsigned long long* total;      // sum over all children (nullptr: skip)
     u32* tile_ctr;       // persistent tile queue counter (zero before launch)
     u32* err;            // set non-zero on capacity overflow
+    u32 tile_parents;    // parents per tile, at most the shape's (0: the shape's); a small level
+                         // uses fewer, so that it spreads over more workgroups
 };
+
+// Parents per tile for a level of n parents on `grid` workgroups of a shape
+// with at most tp_max parents per tile: enough tiles for every workgroup.
+PF_HD u32 level_tile_parents(u64 n, u64 grid, u32 tp_max) {
+    const u64 g = grid ? grid : 1, t = (n + g - 1) / g;
+    return (u32)(t < 1 ? 1 : t > tp_max ? tp_max : t);
+}
 
 enum ExpandMode : int { kLeaf = 0, kWrite = 1 };
 

@@ -106,14 +106,15 @@ template <class SL, int TP, int CAP>
 __device__ __forceinline__ void expand_level(const SL& sl, const LevelArgs& a, const int mode, TileLds<TP, CAP>& s) {
     static_assert(TP <= kWG && (TP & (TP - 1)) == 0, "one parent per thread, power of two");
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const u32 ntiles = (a.n_in + TP - 1) / TP;
+    const u32 tp = (a.tile_parents && a.tile_parents < (u32)TP) ? a.tile_parents : (u32)TP;
+    const u32 ntiles = (a.n_in + tp - 1) / tp;
     for (;;) {
         if (tid == 0) s.tile = atomicAdd(a.tile_ctr, 1u);
         __syncthreads();
         const u32 tile = s.tile;
         if (tile >= ntiles) break;
-        const u32 p0 = tile * TP;
-        const u32 np = min((u32)TP, a.n_in - p0);
+        const u32 p0 = tile * tp;
+        const u32 np = min(tp, a.n_in - p0);
         u32 cnt = 0;
         if (tid < TP) {
             if ((u32)tid < np) {
@@ -307,6 +308,7 @@ __global__ __launch_bounds__(kWG) void k_persistent(const SliderTables* tabs, Pe
         a.tile_ctr = &ctl->tile_ctr[L];
         a.err = &ctl->err;
         a.out_count = &ctl->out_count[L];
+        a.tile_parents = level_tile_parents(n, gridDim.x, PCfg<B, V>::TP);
         if (L == p.depth - 2) {
             a.out = nullptr;
             a.out_root = nullptr;
