@@ -2,17 +2,21 @@
 """Benchmark: leaf nodes/sec of batched and served perft on one MI355X.
 
 Headline ("value"): the C2 batch -- 256 seeded random-playout positions
-(8..40 plies, mate/stalemate discarded), perft depth 5 each, TT off -- in one
-persistent cooperative launch per step (pf_perft_batch_mode(..., PERSISTENT)).
-Positions and counts stay in device memory: a step is the on-device root
-conversion, the launch and the device-side count write-back.  The same steps
-from host buffers (PCIe-inclusive) are reported as "host_buffers".
+(8..40 plies, mate/stalemate discarded), perft depth 5 each, TT off -- as one
+launch per level queued on one stream, every level sized and placed on the
+device (pf_perft_batch_mode(..., STREAM)).  Positions and counts stay in
+device memory: a step is the on-device root conversion, the level launches
+and the device-side count write-back.  The same steps from host buffers
+(PCIe-inclusive) are reported as "host_buffers", and the same batch in one
+persistent cooperative launch (grid barriers between levels) and through the
+host-driven level loop under "backends".
 
 Also reported on the same box, outside the timed region:
-  roofline      integer-VALU roofline of the persistent kernel: algorithmic
-                int32 lane-ops per launch (C2 op model x leaves) / launch time
-                (HIP events on the engine's stream) against MI355X's published
-                vector rate, 157.3 TFLOP/s FP32 = 78.6 T lane-op/s
+  roofline      integer-VALU roofline of the step's kernels: algorithmic int32
+                lane-ops per step (C2 op model x leaves) / the step's kernel
+                time (HIP events on the engine's stream around the level
+                launches) against MI355X's published vector rate,
+                157.3 TFLOP/s FP32 = 78.6 T lane-op/s
   plain         the same 256 positions through the plain per-position path
   backends      batched nodes/sec per slider backend, and the level mode
   serving       C4: Poisson arrivals (50 req/s) replayed in real time for
@@ -263,17 +267,17 @@ def main():
                       "choice": [l.strip() for l in open(tf) if l.startswith("backend " + args.backend + " ")]}
         log("tuning: %s" % tuning)
     # positions and counts resident in HBM: a step is the on-device conversion
-    # of the roots, the persistent launch and the count write-back
+    # of the roots, the level launches and the count write-back
     dpos = pg.DeviceArray.from_host(pos)
     dout = pg.DeviceArray(len(pos), np.uint64)
     for i in range(args.warmup):
-        eng.perft_batch_ptr(dpos.ptr, len(pos), depth, dout.ptr, pg.MODE_PERSISTENT)
+        eng.perft_batch_ptr(dpos.ptr, len(pos), depth, dout.ptr, pg.MODE_STREAM)
         log("warmup %d done" % i)
     eng.sync()
     eng.reset_stats()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        eng.perft_batch_ptr(dpos.ptr, len(pos), depth, dout.ptr, pg.MODE_PERSISTENT)
+        eng.perft_batch_ptr(dpos.ptr, len(pos), depth, dout.ptr, pg.MODE_STREAM)
     eng.sync()
     el = time.perf_counter() - t0
     st = eng.stats()
@@ -282,8 +286,8 @@ def main():
     dout.free()
     leaves = int(counts.sum())
     value = leaves * args.steps / el
-    # per step: one root-conversion kernel and one persistent launch; gpu_ms
-    # holds the persistent launches only
+    # per step: the root conversion, then the root count and depth - 1 level
+    # launches; gpu_ms holds the span from the root count to the last level
     kernel_ms = st["gpu_ms"] / args.steps
     ops = W.C2_OPS_PER_LEAF * leaves
     achieved = ops / (kernel_ms * 1e-3) / 1e12
@@ -303,7 +307,7 @@ def main():
         "dtype": "u64",
         "data": "synthetic: seeded random legal playouts from the initial position",
         "config": {"workload": cfg["name"], "positions": len(pos), "depth": depth, "tt": "off",
-                   "backend": args.backend, "mode": "persistent", "buffers": "device",
+                   "backend": args.backend, "mode": "stream", "buffers": "device",
                    "leaves_per_step": leaves,
                    "counts_match_golden": leaves == W.C2_LEAVES, "tuning": tuning},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
@@ -311,7 +315,8 @@ def main():
                      # HBM bytes per launch: FETCH_SIZE x2.00 + WRITE_SIZE x1.00 (rocprofv3 --pmc, calibrated
                      # on a copy of known size; profiles/k_persistent_ks_c2_d5_counters.md)
                      "traffic": 9.133e8,
-                     "kernel": "k_persistent<%s>" % args.backend, "kernel_ms": kernel_ms,
+                     "kernel": "k_stream_roots + %d x k_stream_level<%s>" % (depth - 1, args.backend),
+                     "kernel_ms": kernel_ms,
                      "ops_per_leaf": W.C2_OPS_PER_LEAF},
     }
 
@@ -321,7 +326,7 @@ def main():
         # upload and the 2 KiB count read-back over PCIe on every step
         t = time.perf_counter()
         for _ in range(args.steps):
-            c = eng.perft_batch(pos, depth, mode=pg.MODE_PERSISTENT)
+            c = eng.perft_batch(pos, depth, mode=pg.MODE_STREAM)
         th = time.perf_counter() - t
         line["host_buffers"] = {"value": leaves * args.steps / th, "unit": "nodes/s",
                                 "ms_per_step": th * 1e3 / args.steps, "counts_match": bool((c == counts).all())}
@@ -337,18 +342,21 @@ def main():
         be = {}
         for b in ("ks", "plain", "fancy", "byte"):
             eng.set_backend(b)
-            eng.perft_batch(pos, depth, mode=pg.MODE_PERSISTENT)
+            eng.perft_batch(pos, depth, mode=pg.MODE_STREAM)
             t = time.perf_counter()
             reps = 3
             for _ in range(reps):
-                c = eng.perft_batch(pos, depth, mode=pg.MODE_PERSISTENT)
+                c = eng.perft_batch(pos, depth, mode=pg.MODE_STREAM)
             be[b] = {"value": leaves * reps / (time.perf_counter() - t), "counts_match": bool((c == counts).all())}
             log("backend %s: %.3e nodes/s" % (b, be[b]["value"]))
         eng.set_backend(args.backend)
-        t = time.perf_counter()
-        c = eng.perft_batch(pos, depth, mode=pg.MODE_LEVELS)
-        be["levels_mode_" + args.backend] = {"value": leaves / (time.perf_counter() - t),
-                                             "counts_match": bool((c == counts).all())}
+        for name, mode in (("persistent_mode_", pg.MODE_PERSISTENT), ("levels_mode_", pg.MODE_LEVELS)):
+            eng.perft_batch(pos, depth, mode=mode)
+            t = time.perf_counter()
+            for _ in range(3):
+                c = eng.perft_batch(pos, depth, mode=mode)
+            be[name + args.backend] = {"value": leaves * 3 / (time.perf_counter() - t),
+                                       "counts_match": bool((c == counts).all())}
         line["backends"] = be
     eng.close()
 

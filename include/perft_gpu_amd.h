@@ -57,10 +57,13 @@ typedef enum pf_backend {
 } pf_backend;
 
 typedef enum pf_mode {
-    PF_MODE_AUTO = 0,        /* TT configured: levels with the table; else persistent
-                                single launch when it fits, else levels */
+    PF_MODE_AUTO = 0,        /* TT configured: levels with the table; else the stream-ordered
+                                single pass (PF_MODE_STREAM) when it fits, else levels */
     PF_MODE_LEVELS = 1,      /* host-driven level loop with depth-first chunking */
-    PF_MODE_PERSISTENT = 2   /* one cooperative launch for the whole batch (no TT) */
+    PF_MODE_PERSISTENT = 2,  /* one cooperative launch for the whole batch (no TT) */
+    PF_MODE_STREAM = 3       /* one launch per level queued on the stream, every level sized and
+                                placed on the device: no host round trip, no grid barrier (no TT);
+                                PF_ENOMEM when the batch does not fit the arena unchunked */
 } pf_mode;
 
 typedef struct pf_config {
@@ -120,7 +123,8 @@ pf_status pf_perft(pf_ctx *ctx, const pf_pos48 *pos, int depth, uint64_t *out);
  * (perft_bb.h:1242), which needs one side to move and <= 256 roots.
  * `pos` and `out` may each be host memory or device memory of the context's
  * device (as in the reference, whose kernel takes device arrays).  With both
- * on the device, a table-free run (PF_MODE_PERSISTENT, or AUTO with no table)
+ * on the device, a table-free run (PF_MODE_STREAM, PF_MODE_PERSISTENT, or AUTO
+ * with no table)
  * converts the roots on the device and leaves the counts there: no host
  * staging.  Other combinations stage through host memory.  A device pointer
  * of another device is PF_EINVAL. */

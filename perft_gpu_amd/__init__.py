@@ -21,7 +21,7 @@ CLI_PATH = os.path.join(HERE, "_build", "perft_gpu")
 
 PF_OK, PF_EINVAL, PF_ENOMEM, PF_EDEVICE, PF_EBUSY, PF_EAGAIN, PF_EUNAVAIL, PF_EIO = 0, -1, -2, -3, -4, -5, -6, -7
 BACKENDS = {"ks": 0, "plain": 1, "fancy": 2, "byte": 3}
-MODE_AUTO, MODE_LEVELS, MODE_PERSISTENT = 0, 1, 2
+MODE_AUTO, MODE_LEVELS, MODE_PERSISTENT, MODE_STREAM = 0, 1, 2, 3
 START_FEN = "rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1"
 
 # structured dtype with the exact pf_pos48 layout, for zero-copy batches

@@ -206,19 +206,28 @@ static int buffer_kind(const void* p, int device) {
     return a.device == device ? 1 : -1;
 }
 
+// The table-free single passes: PERSISTENT the cooperative launch, STREAM
+// (and AUTO) the stream-ordered level launches.
+static pf_status single_pass(pf_ctx* ctx, const pf_pos48* pos, int n, int depth, int mode, uint64_t* out, bool din,
+                             bool dout) {
+    if (mode == PF_MODE_PERSISTENT) return ctx->eng.perft_persistent(pos, n, depth, out, din, dout);
+    return ctx->eng.perft_stream(pos, n, depth, out, din, dout);
+}
+
 static pf_status batch_host(pf_ctx* ctx, const pf_pos48* pos, int n, int depth, int mode, uint64_t* out) {
     // the transposition table lives in the level driver; AUTO uses it when
-    // configured, PERSISTENT always runs the single table-free launch
+    // configured, else a single pass that falls back to the level driver
     if (mode == PF_MODE_LEVELS || (mode == PF_MODE_AUTO && ctx->eng.d_tt && depth >= 3))
         return ctx->eng.perft_levels(pos, n, depth, out);
-    pf_status s = ctx->eng.perft_persistent(pos, n, depth, out);
+    pf_status s = single_pass(ctx, pos, n, depth, mode, out, false, false);
     if ((s == PF_ENOMEM || s == PF_EUNAVAIL) && mode == PF_MODE_AUTO) s = ctx->eng.perft_levels(pos, n, depth, out);
     return s;
 }
 
 pf_status pf_perft_batch_mode(pf_ctx* ctx, const pf_pos48* pos, int n, int depth, int mode, uint64_t* out) {
     if (!ctx || (n > 0 && (!pos || !out)) || n < 0 || check_depth(depth)) return PF_EINVAL;
-    if (mode != PF_MODE_AUTO && mode != PF_MODE_LEVELS && mode != PF_MODE_PERSISTENT) return PF_EINVAL;
+    if (mode != PF_MODE_AUTO && mode != PF_MODE_LEVELS && mode != PF_MODE_PERSISTENT && mode != PF_MODE_STREAM)
+        return PF_EINVAL;
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
     if (ctx->eng.serving) return PF_EBUSY;
     if (n == 0) return PF_OK;
@@ -228,7 +237,7 @@ pf_status pf_perft_batch_mode(pf_ctx* ctx, const pf_pos48* pos, int n, int depth
     if (!kin && !kout) return batch_host(ctx, pos, n, depth, mode, out);
     const bool table_run = mode == PF_MODE_LEVELS || (mode == PF_MODE_AUTO && ctx->eng.d_tt && depth >= 3);
     if (kin && kout && !table_run && depth >= 2) {
-        pf_status s = ctx->eng.perft_persistent(pos, n, depth, out, true, true);
+        pf_status s = single_pass(ctx, pos, n, depth, mode, out, true, true);
         if (!((s == PF_ENOMEM || s == PF_EUNAVAIL) && mode == PF_MODE_AUTO)) return s;
     }
     // staged: device buffers round-trip through host memory around the host path

@@ -67,7 +67,10 @@ pf_status Engine::init(const pf_config& cfg) {
     PF_HIP(hipEventCreate(&ev1));
     PF_HIP(hipMalloc(&d_pctl, sizeof(PersistCtl)));
     for (int b = 0; b < 4; b++)
-        for (int m = 0; m < 2; m++) grid[b][m] = expand_grid(b, m, n_cu);
+        for (int m = 0; m < 2; m++) {
+            grid[b][m] = expand_grid(b, m, n_cu);
+            sgrid[b][m] = stream_grid(b, m, n_cu);
+        }
     apply_tuning();
     return ensure_sums(1024);
 }
@@ -265,6 +268,55 @@ pf_status Engine::perft_levels(const pf_pos48* pos, int n, int d, u64* out) {
     st.last_expand_ms = run_expand_launches ? run_expand_ms / (double)run_expand_launches : 0.0;
     st.last_expand_launches = run_expand_launches;
     release(mark);
+    return PF_OK;
+}
+
+pf_status Engine::perft_stream(const pf_pos48* pos, int n, int d, u64* out, bool dev_in, bool dev_out) {
+    if (n <= 0) return PF_OK;
+    if (d <= 1) {
+        if (dev_in || dev_out) return PF_EINVAL;  // callers stage shallow device batches themselves
+        return perft_levels(pos, n, d, out);
+    }
+    PF_HIP(hipSetDevice(device));
+    pf_status s = ensure_sums((u64)n);
+    if (s != PF_OK) return s;
+    char* mark = arena + arena_top;
+    auto fail = [&](pf_status e) { release(mark); return e; };
+    Board* roots = nullptr;
+    u32* rid = nullptr;
+    s = upload_roots(pos, n, &roots, &rid, dev_in);
+    if (s != PF_OK) return fail(s);
+    if (hipMemsetAsync(d_pctl, 0, sizeof(PersistCtl), stream) != hipSuccess) return fail(PF_EDEVICE);
+    PersistArgs p;
+    p.roots = roots;
+    p.root_ids = rid;
+    p.n_roots = (u32)n;
+    p.depth = d;
+    p.arena = arena + arena_top;
+    p.arena_bytes = arena_bytes - arena_top;
+    p.ctl = d_pctl;
+    p.sums = d_sums;
+    if (timing && hipEventRecord(ev0, stream) != hipSuccess) return fail(PF_EDEVICE);
+    if (launch_stream(backend, DevTables{d_tab}, p, sgrid[backend], stream) != hipSuccess) return fail(PF_EDEVICE);
+    if (timing && hipEventRecord(ev1, stream) != hipSuccess) return fail(PF_EDEVICE);
+    st.kernel_launches += (u64)d;  // the root count and d - 1 levels
+    st.levels_run += (u64)(d - 1);
+    PersistCtl h;
+    if (hipMemcpyAsync(&h, d_pctl, sizeof(PersistCtl), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipMemcpyAsync(out, d_sums, (size_t)n * 8, dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                       stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return fail(PF_EDEVICE);
+    if (timing) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, ev0, ev1);
+        st.gpu_ms += ms;
+        st.last_expand_ms = ms;
+        st.last_expand_launches = 1;
+    }
+    release(mark);
+    if (h.err & 2) return PF_ENOMEM;
+    if (h.err) return PF_EDEVICE;
     return PF_OK;
 }
 

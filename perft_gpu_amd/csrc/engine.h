@@ -35,6 +35,7 @@ struct Engine {
 
     int grid[4][2] = {};
     int pgrid[4] = {};
+    int sgrid[4][2] = {};  // stream-ordered level launches: [backend][kLeaf / kWrite]
     // launch parameters per backend, chosen by autotune() or a tune file
     // (0 blocks per CU = as many as can be resident)
     struct Tune {
@@ -85,6 +86,9 @@ struct Engine {
     // dev_in / dev_out: `pos` / `out` are device memory of this engine's device
     pf_status perft_persistent(const pf_pos48* pos, int n, int depth, u64* out, bool dev_in = false,
                                bool dev_out = false);
+    // the same batch as one launch per level on the stream, sized on the device
+    pf_status perft_stream(const pf_pos48* pos, int n, int depth, u64* out, bool dev_in = false,
+                           bool dev_out = false);
     pf_status init_tt(u64 bytes, u64 seed);
     void apply_tuning();
     pf_status load_tuning(const char* path);

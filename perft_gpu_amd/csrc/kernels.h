@@ -57,6 +57,7 @@ struct PersistCtl {
     u32 err;
     u32 pad[3];
     unsigned long long totals[20];  // totals[L] = number of nodes of level L
+    unsigned long long lvl_off[20];  // stream-ordered form: arena offset of level L
 };
 struct PersistArgs {
     Board* roots;
@@ -74,5 +75,10 @@ constexpr int kPersistVariants = 3;
 int persistent_max_bpc(int backend, int variant);
 hipError_t launch_persistent(int backend, int variant, const DevTables& dt, const PersistArgs& a, int grid,
                              hipStream_t s);
+// The same batch as one launch per level on stream s (levels sized and placed
+// on the device; no host round trip): the root count, then depth - 1 level
+// launches of grid[kWrite] / grid[kLeaf] workgroups.
+int stream_grid(int backend, int mode, int n_cu);
+hipError_t launch_stream(int backend, const DevTables& dt, const PersistArgs& a, const int grid[2], hipStream_t s);
 
 }  // namespace pf

@@ -340,6 +340,81 @@ __global__ __launch_bounds__(kWG) void k_persistent(const SliderTables* tabs, Pe
     }
 }
 
+// The same batch as a sequence of launches on one stream, one per level: the
+// levels are sized and placed on the device exactly as in k_persistent (the
+// totals and level offsets live in PersistCtl), so nothing returns to the
+// host between levels, and the launch boundary replaces the grid barrier.
+// Every workgroup of a level launch derives the same placement from words
+// the previous launch finished writing.
+template <int B>
+__global__ __launch_bounds__(kWG) void k_stream_roots(const SliderTables* tabs, PersistArgs p) {
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    const auto sl = stage_backend<B>(tabs, dyn_lds);
+    const u32 i = blockIdx.x * kWG + threadIdx.x;
+    unsigned long long c = 0;
+    if (i < p.n_roots) {
+        const Board b = p.roots[i];
+        c = count_moves(b, sl);
+        p.roots[i] = with_count(b, (u32)c);
+        if (p.depth == 1) p.sums[p.root_ids[i]] = c;
+    }
+    c = wave_sum64(c);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&p.ctl->totals[1], c);
+}
+
+template <int B, int MODE>
+__global__ __launch_bounds__(kWG) void k_stream_level(const SliderTables* tabs, PersistArgs p, int L) {
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    __shared__ TileLds<PCfg<B, 0>::TP, PCfg<B, 0>::CAP> s;
+    PersistCtl* ctl = p.ctl;
+    // an earlier level did not fit: nothing below it is counted (one read per
+    // workgroup, so that the whole workgroup leaves together)
+    if (threadIdx.x == 0) s.ok = ctl->err;
+    __syncthreads();
+    if (s.ok) return;
+    const auto sl = stage_backend<B>(tabs, dyn_lds);
+    const Board* cur = p.roots;
+    const u32* cur_r = p.root_ids;
+    u64 n = p.n_roots, off = 0;
+    if (L > 0) {
+        n = ctl->totals[L];
+        off = ctl->lvl_off[L];
+        cur = reinterpret_cast<const Board*>(p.arena + off);
+        cur_r = reinterpret_cast<const u32*>(p.arena + off + ((n * sizeof(Board) + 255) & ~255ULL));
+        off += ((n * sizeof(Board) + 255) & ~255ULL) + ((n * sizeof(u32) + 255) & ~255ULL);
+    }
+    LevelArgs a;
+    a.in = cur;
+    a.in_root = cur_r;
+    a.n_in = (u32)n;
+    a.tile_ctr = &ctl->tile_ctr[L];
+    a.err = &ctl->err;
+    a.out_count = &ctl->out_count[L];
+    a.tile_parents = level_tile_parents(n, gridDim.x, PCfg<B, 0>::TP);
+    if (MODE == kLeaf) {
+        a.out = nullptr;
+        a.out_root = nullptr;
+        a.out_cap = 0;
+        a.root_sums = p.sums;
+        a.total = nullptr;
+        expand_level(sl, a, kLeaf, s);
+        return;
+    }
+    const u64 nn = ctl->totals[L + 1];
+    const u64 bbytes = (nn * sizeof(Board) + 255) & ~255ULL, rbytes = (nn * sizeof(u32) + 255) & ~255ULL;
+    if (nn > 0xFFFFFFFFull || off + bbytes + rbytes > p.arena_bytes) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&ctl->err, 2u);
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->lvl_off[L + 1] = off;
+    a.out = reinterpret_cast<Board*>(p.arena + off);
+    a.out_root = reinterpret_cast<u32*>(p.arena + off + bbytes);
+    a.out_cap = nn;
+    a.root_sums = nullptr;
+    a.total = &ctl->totals[L + 2];
+    expand_level(sl, a, kWrite, s);
+}
+
 // ---- launch plumbing --------------------------------------------------------
 template <int B, int MODE>
 static void* expand_fn() {
@@ -415,6 +490,49 @@ hipError_t launch_persistent(int backend, int variant, const DevTables& dt, cons
     const SliderTables* t = dt.t;
     void* kargs[] = {(void*)&t, (void*)&args};
     return hipLaunchCooperativeKernel(f, dim3(grid), dim3(kWG), kargs, (unsigned)dyn_bytes(backend), s);
+}
+
+template <int B>
+static void* stream_level_fn(int mode) {
+    return mode ? reinterpret_cast<void*>(&k_stream_level<B, kWrite>) : reinterpret_cast<void*>(&k_stream_level<B, kLeaf>);
+}
+static void* pick_stream_level(int backend, int mode) {
+    switch (backend) {
+    case kKS: return stream_level_fn<kKS>(mode);
+    case kPlain: return stream_level_fn<kPlain>(mode);
+    case kFancy: return stream_level_fn<kFancy>(mode);
+    default: return stream_level_fn<kByte>(mode);
+    }
+}
+static void* pick_stream_roots(int backend) {
+    switch (backend) {
+    case kKS: return reinterpret_cast<void*>(&k_stream_roots<kKS>);
+    case kPlain: return reinterpret_cast<void*>(&k_stream_roots<kPlain>);
+    case kFancy: return reinterpret_cast<void*>(&k_stream_roots<kFancy>);
+    default: return reinterpret_cast<void*>(&k_stream_roots<kByte>);
+    }
+}
+
+int stream_grid(int backend, int mode, int n_cu) { return blocks_per_cu(pick_stream_level(backend, mode), backend) * n_cu; }
+
+hipError_t launch_stream(int backend, const DevTables& dt, const PersistArgs& a, const int grid[2], hipStream_t s) {
+    PersistArgs args = a;
+    const SliderTables* t = dt.t;
+    const unsigned dyn = (unsigned)dyn_bytes(backend);
+    if (backend == kByte) hipFuncSetAttribute(pick_stream_roots(backend), hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    {
+        void* kargs[] = {(void*)&t, (void*)&args};
+        const hipError_t e = hipLaunchKernel(pick_stream_roots(backend), dim3((a.n_roots + kWG - 1) / kWG), dim3(kWG),
+                                             kargs, dyn, s);
+        if (e != hipSuccess) return e;
+    }
+    for (int L = 0; L <= a.depth - 2; L++) {
+        const int mode = (L == a.depth - 2) ? kLeaf : kWrite;
+        void* kargs[] = {(void*)&t, (void*)&args, (void*)&L};
+        const hipError_t e = hipLaunchKernel(pick_stream_level(backend, mode), dim3(grid[mode]), dim3(kWG), kargs, dyn, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_range_count(const Board* b, u32 n, unsigned long long* out, hipStream_t s) {

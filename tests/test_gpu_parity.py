@@ -35,7 +35,7 @@ def test_cpw_known_answers(pg, engine, backend):
         engine.set_backend("ks")
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_batch_vs_oracle(pg, oracle, engine, mode):
     arr = pg.playouts(20240601, 96, 8, 60)
     for depth in (1, 2, 3, 4):
@@ -56,7 +56,7 @@ def _rows(pg, hexrows):
     return arr
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_c2_full_size_vs_golden(pg, engine, mode):
     """the full C2 workload (256 positions, perft 5, 9.0e9 leaves)"""
     g = _golden()["c2"]
@@ -72,10 +72,11 @@ def test_c2_backends_vs_golden(pg, engine, backend):
     pos = _rows(pg, g["positions"])
     engine.set_backend(backend)
     try:
-        got = engine.perft_batch(pos, 5, mode=2)
+        for mode in (pg.MODE_PERSISTENT, pg.MODE_STREAM):
+            got = engine.perft_batch(pos, 5, mode=mode)
+            assert [int(x) for x in got] == g["perft"]["5"], mode
     finally:
         engine.set_backend("ks")
-    assert [int(x) for x in got] == g["perft"]["5"]
 
 
 def test_c3_c5_positions_vs_golden(pg, engine):
@@ -113,10 +114,11 @@ def test_chunking_small_arena(pg, oracle):
         got = eng.perft_batch(pos, 5, mode=pg.MODE_LEVELS)
         assert np.array_equal(got, want)
         assert eng.stats()["chunks"] > 0
-        # persistent mode cannot chunk: it reports ENOMEM, AUTO falls back to levels
-        with pytest.raises(pg.PerftError) as ei:
-            eng.perft_batch(pos, 5, mode=pg.MODE_PERSISTENT)
-        assert ei.value.status == pg.PF_ENOMEM
+        # the single passes cannot chunk: they report ENOMEM, AUTO falls back to levels
+        for mode in (pg.MODE_PERSISTENT, pg.MODE_STREAM):
+            with pytest.raises(pg.PerftError) as ei:
+                eng.perft_batch(pos, 5, mode=mode)
+            assert ei.value.status == pg.PF_ENOMEM, mode
         assert np.array_equal(eng.perft_batch(pos, 5, mode=pg.MODE_AUTO), want)
     finally:
         eng.close()
@@ -171,7 +173,7 @@ def test_device_resident_batches(pg, oracle, engine):
     dout = pg.DeviceArray(len(pos), np.uint64)
     hout = np.zeros(len(pos), dtype=np.uint64)
     try:
-        for mode in (pg.MODE_AUTO, pg.MODE_PERSISTENT, pg.MODE_LEVELS):
+        for mode in (pg.MODE_AUTO, pg.MODE_PERSISTENT, pg.MODE_STREAM, pg.MODE_LEVELS):
             for d in (1, 2, 4):
                 dout.upload(np.full(len(pos), 7, dtype=np.uint64))
                 engine.perft_batch_ptr(dpos.ptr, len(pos), d, dout.ptr, mode)
