@@ -312,9 +312,10 @@ def main():
                    "counts_match_golden": leaves == W.C2_LEAVES, "tuning": tuning},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
                      "frac": achieved / VALU_PEAK_TOPS,
-                     # HBM bytes per launch: FETCH_SIZE x2.00 + WRITE_SIZE x1.00 (rocprofv3 --pmc, calibrated
-                     # on a copy of known size; profiles/k_persistent_ks_c2_d5_counters.md)
-                     "traffic": 9.133e8,
+                     # HBM bytes per step: FETCH_SIZE x2.00 + WRITE_SIZE x1.00 over the step's kernels
+                     # (rocprofv3 --pmc, calibrated on a copy of known size;
+                     # profiles/k_stream_level_ks_c2_d5_counters.md)
+                     "traffic": 8.993e8,
                      "kernel": "k_stream_roots + %d x k_stream_level<%s>" % (depth - 1, args.backend),
                      "kernel_ms": kernel_ms,
                      "ops_per_leaf": W.C2_OPS_PER_LEAF},
