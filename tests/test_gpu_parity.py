@@ -96,9 +96,14 @@ def test_edge_cases(pg, oracle, engine):
     mate = pg.fen_to_pos("rnb1kbnr/pppp1ppp/8/4p3/6Pq/5P2/PPPPP2P/RNBQKBNR w KQkq - 1 3")
     stale = pg.fen_to_pos("7k/5Q2/6K1/8/8/8/8/8 b - - 0 1")
     m218 = pg.fen_to_pos("3Q4/1Q4Q1/4Q3/2Q4R/Q4Q2/3Q4/1Q4Rp/1K1BBNNk w - - 0 1")
-    for d in (1, 2, 3, 4):
-        got = engine.perft_batch([mate, stale, start], d, mode=2)
-        assert list(map(int, got[:2])) == [0, 0]
+    startpos = [1, 20, 400, 8902, 197281]
+    for mode in (pg.MODE_PERSISTENT, pg.MODE_STREAM):
+        for d in (1, 2, 3, 4):
+            got = engine.perft_batch([mate, stale, start], d, mode=mode)
+            assert list(map(int, got)) == [0, 0, startpos[d]], (mode, d)
+        # a batch whose every root is mate or stalemate: empty levels below the roots
+        for d in (2, 3, 4):
+            assert list(map(int, engine.perft_batch([mate, stale], d, mode=mode))) == [0, 0], (mode, d)
     assert engine.perft(m218, 1) == 218
     assert engine.perft(m218, 3) == oracle.perft_fen("3Q4/1Q4Q1/4Q3/2Q4R/Q4Q2/3Q4/1Q4Rp/1K1BBNNk w - - 0 1", 3)
     with pytest.raises(pg.PerftError):
