@@ -4,7 +4,9 @@
 Headline ("value"): the C2 batch -- 256 seeded random-playout positions
 (8..40 plies, mate/stalemate discarded), perft depth 5 each, TT off -- as one
 launch per level queued on one stream, every level sized and placed on the
-device (pf_perft_batch_mode(..., STREAM)).  Positions and counts stay in
+device (pf_perft_batch_mode(..., AUTO) after the autotuner, which picks
+this form or the cooperative launch; the one used is in config.mode).
+Positions and counts stay in
 device memory: a step is the on-device root conversion, the level launches
 and the device-side count write-back.  The same steps from host buffers
 (PCIe-inclusive) are reported as "host_buffers", and the same batch in one
@@ -266,18 +268,22 @@ def main():
             tuning = {"seconds": time.perf_counter() - t,
                       "choice": [l.strip() for l in open(tf) if l.startswith("backend " + args.backend + " ")]}
         log("tuning: %s" % tuning)
+    # the single pass PF_MODE_AUTO runs for this backend (stream until tuned)
+    single = "stream"
+    if tuning and tuning["choice"] and "single_pass persistent" in tuning["choice"][0]:
+        single = "persistent"
     # positions and counts resident in HBM: a step is the on-device conversion
     # of the roots, the level launches and the count write-back
     dpos = pg.DeviceArray.from_host(pos)
     dout = pg.DeviceArray(len(pos), np.uint64)
     for i in range(args.warmup):
-        eng.perft_batch_ptr(dpos.ptr, len(pos), depth, dout.ptr, pg.MODE_STREAM)
+        eng.perft_batch_ptr(dpos.ptr, len(pos), depth, dout.ptr, pg.MODE_AUTO)
         log("warmup %d done" % i)
     eng.sync()
     eng.reset_stats()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        eng.perft_batch_ptr(dpos.ptr, len(pos), depth, dout.ptr, pg.MODE_STREAM)
+        eng.perft_batch_ptr(dpos.ptr, len(pos), depth, dout.ptr, pg.MODE_AUTO)
     eng.sync()
     el = time.perf_counter() - t0
     st = eng.stats()
@@ -307,7 +313,7 @@ def main():
         "dtype": "u64",
         "data": "synthetic: seeded random legal playouts from the initial position",
         "config": {"workload": cfg["name"], "positions": len(pos), "depth": depth, "tt": "off",
-                   "backend": args.backend, "mode": "stream", "buffers": "device",
+                   "backend": args.backend, "mode": "auto: " + single, "buffers": "device",
                    "leaves_per_step": leaves,
                    "counts_match_golden": leaves == W.C2_LEAVES, "tuning": tuning},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
@@ -316,7 +322,8 @@ def main():
                      # (rocprofv3 --pmc, calibrated on a copy of known size;
                      # profiles/k_stream_level_ks_c2_d5_counters.md)
                      "traffic": 8.993e8,
-                     "kernel": "k_stream_roots + %d x k_stream_level<%s>" % (depth - 1, args.backend),
+                     "kernel": ("k_stream_roots + %d x k_stream_level<%s>" % (depth - 1, args.backend)
+                                if single == "stream" else "k_persistent<%s>" % args.backend),
                      "kernel_ms": kernel_ms,
                      "ops_per_leaf": W.C2_OPS_PER_LEAF},
     }
@@ -327,7 +334,7 @@ def main():
         # upload and the 2 KiB count read-back over PCIe on every step
         t = time.perf_counter()
         for _ in range(args.steps):
-            c = eng.perft_batch(pos, depth, mode=pg.MODE_STREAM)
+            c = eng.perft_batch(pos, depth, mode=pg.MODE_AUTO)
         th = time.perf_counter() - t
         line["host_buffers"] = {"value": leaves * args.steps / th, "unit": "nodes/s",
                                 "ms_per_step": th * 1e3 / args.steps, "counts_match": bool((c == counts).all())}
@@ -343,15 +350,16 @@ def main():
         be = {}
         for b in ("ks", "plain", "fancy", "byte"):
             eng.set_backend(b)
-            eng.perft_batch(pos, depth, mode=pg.MODE_STREAM)
+            eng.perft_batch(pos, depth, mode=pg.MODE_AUTO)
             t = time.perf_counter()
             reps = 3
             for _ in range(reps):
-                c = eng.perft_batch(pos, depth, mode=pg.MODE_STREAM)
+                c = eng.perft_batch(pos, depth, mode=pg.MODE_AUTO)
             be[b] = {"value": leaves * reps / (time.perf_counter() - t), "counts_match": bool((c == counts).all())}
             log("backend %s: %.3e nodes/s" % (b, be[b]["value"]))
         eng.set_backend(args.backend)
-        for name, mode in (("persistent_mode_", pg.MODE_PERSISTENT), ("levels_mode_", pg.MODE_LEVELS)):
+        for name, mode in (("stream_mode_", pg.MODE_STREAM), ("persistent_mode_", pg.MODE_PERSISTENT),
+                           ("levels_mode_", pg.MODE_LEVELS)):
             eng.perft_batch(pos, depth, mode=mode)
             t = time.perf_counter()
             for _ in range(3):

@@ -206,11 +206,14 @@ static int buffer_kind(const void* p, int device) {
     return a.device == device ? 1 : -1;
 }
 
-// The table-free single passes: PERSISTENT the cooperative launch, STREAM
-// (and AUTO) the stream-ordered level launches.
+// The table-free single passes: PERSISTENT the cooperative launch, STREAM the
+// stream-ordered level launches, AUTO the one the tuner chose (stream until
+// tuned).
 static pf_status single_pass(pf_ctx* ctx, const pf_pos48* pos, int n, int depth, int mode, uint64_t* out, bool din,
                              bool dout) {
-    if (mode == PF_MODE_PERSISTENT) return ctx->eng.perft_persistent(pos, n, depth, out, din, dout);
+    const bool coop = mode == PF_MODE_PERSISTENT ||
+                      (mode == PF_MODE_AUTO && ctx->eng.tune[ctx->eng.backend].single_pass == 1);
+    if (coop) return ctx->eng.perft_persistent(pos, n, depth, out, din, dout);
     return ctx->eng.perft_stream(pos, n, depth, out, din, dout);
 }
 

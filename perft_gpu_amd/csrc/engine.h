@@ -39,9 +39,10 @@ struct Engine {
     // launch parameters per backend, chosen by autotune() or a tune file
     // (0 blocks per CU = as many as can be resident)
     struct Tune {
-        int p_variant = 0;  // persistent kernel tile shape (kernels.h)
-        int p_bpc = 0;      // persistent kernel workgroups per CU
-        int tt_bpc = 0;     // TT expand kernel workgroups per CU
+        int p_variant = 0;    // persistent kernel tile shape (kernels.h)
+        int p_bpc = 0;        // persistent kernel workgroups per CU
+        int tt_bpc = 0;       // TT expand kernel workgroups per CU
+        int single_pass = 0;  // table-free batches under PF_MODE_AUTO: 0 stream-ordered levels, 1 cooperative
     };
     Tune tune[4];
     PersistCtl* d_pctl = nullptr;

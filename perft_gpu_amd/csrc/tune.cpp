@@ -23,6 +23,7 @@
 namespace pf {
 
 static const char* const kBackendName[4] = {"ks", "plain", "fancy", "byte"};
+static const char* const kSinglePassName[2] = {"stream", "persistent"};
 
 void Engine::apply_tuning() {
     for (int b = 0; b < 4; b++) {
@@ -46,16 +47,20 @@ pf_status Engine::load_tuning(const char* path) {
     pf_status s = PF_OK;
     if (!fgets(line, sizeof line, f) || strncmp(line, "perft_gpu_amd tuning 1", 22) != 0) s = PF_EINVAL;
     while (s == PF_OK && fgets(line, sizeof line, f)) {
-        char name[16];
+        char name[16], sp[16] = "stream";
         int v = 0, pb = 0, tb = 0;
         if (strncmp(line, "backend ", 8) != 0) continue;
-        if (sscanf(line, "backend %15s persistent_variant %d persistent_blocks_per_cu %d tt_blocks_per_cu %d", name, &v,
-                   &pb, &tb) != 4) {
+        const int got = sscanf(line,
+                               "backend %15s persistent_variant %d persistent_blocks_per_cu %d tt_blocks_per_cu %d "
+                               "single_pass %15s",
+                               name, &v, &pb, &tb, sp);
+        const int single = strcmp(sp, kSinglePassName[1]) == 0 ? 1 : strcmp(sp, kSinglePassName[0]) == 0 ? 0 : -1;
+        if (got < 4 || single < 0) {
             s = PF_EINVAL;
             break;
         }
         for (int b = 0; b < 4; b++)
-            if (strcmp(name, kBackendName[b]) == 0) tune[b] = Tune{v, pb, tb};
+            if (strcmp(name, kBackendName[b]) == 0) tune[b] = Tune{v, pb, tb, single};
     }
     fclose(f);
     apply_tuning();
@@ -67,15 +72,17 @@ pf_status Engine::save_tuning(const char* path) const {
     if (!f) return PF_EINVAL;
     fprintf(f, "perft_gpu_amd tuning 1\ncus %d\n", n_cu);
     for (int b = 0; b < 4; b++)
-        fprintf(f, "backend %s persistent_variant %d persistent_blocks_per_cu %d tt_blocks_per_cu %d\n",
-                kBackendName[b], tune[b].p_variant, tune[b].p_bpc, tune[b].tt_bpc);
+        fprintf(f, "backend %s persistent_variant %d persistent_blocks_per_cu %d tt_blocks_per_cu %d single_pass %s\n",
+                kBackendName[b], tune[b].p_variant, tune[b].p_bpc, tune[b].tt_bpc,
+                kSinglePassName[tune[b].single_pass ? 1 : 0]);
     return fclose(f) == 0 ? PF_OK : PF_EINVAL;
 }
 
 // Times the current backend on the sample (positions, depth): every tile shape
-// x workgroups-per-CU of the persistent kernel (kernel time from HIP events,
-// best of three after a warm-up), then, with a table, the TT expand kernel's
-// workgroups per CU (whole level-driver time, table cleared before each run).
+// x workgroups-per-CU of the persistent kernel and the stream-ordered level
+// launches (kernel time from HIP events, best of three after a warm-up), then,
+// with a table, the TT expand kernel's workgroups per CU (whole level-driver
+// time, table cleared before each run).
 pf_status Engine::autotune(const pf_pos48* pos, int n, int d) {
     if (!pos || n <= 0 || d < 2 || d > 16) return PF_EINVAL;
     std::vector<u64> out((size_t)n), want((size_t)n);
@@ -83,10 +90,10 @@ pf_status Engine::autotune(const pf_pos48* pos, int n, int d) {
     const Tune keep = tune[b];
     pf_status s = PF_OK;
     bool have_want = false;
-    auto time_persistent = [&](double* ms) -> pf_status {
+    auto time_pass = [&](bool cooperative, double* ms) -> pf_status {
         double best = 1e30;
         for (int r = 0; r < 4; r++) {
-            pf_status e = perft_persistent(pos, n, d, out.data());
+            pf_status e = cooperative ? perft_persistent(pos, n, d, out.data()) : perft_stream(pos, n, d, out.data());
             if (e != PF_OK) return e;
             if (!have_want) {
                 want = out;
@@ -111,7 +118,7 @@ pf_status Engine::autotune(const pf_pos48* pos, int n, int d) {
             tune[b].p_bpc = pb;
             apply_tuning();
             double ms = 0;
-            s = time_persistent(&ms);
+            s = time_pass(true, &ms);
             if (s == PF_ENOMEM || s == PF_EUNAVAIL) {  // the sample does not fit, or no cooperative launch now
                 s = PF_OK;
                 break;
@@ -126,6 +133,17 @@ pf_status Engine::autotune(const pf_pos48* pos, int n, int d) {
     }
     tune[b].p_variant = best.p_variant;
     tune[b].p_bpc = best.p_bpc;
+    apply_tuning();
+    if (s == PF_OK) {  // the stream-ordered form against the best cooperative shape
+        double ms = 0;
+        s = time_pass(false, &ms);
+        if (s == PF_ENOMEM) {  // does not fit unchunked: neither does the cooperative launch
+            s = PF_OK;
+        } else if (s == PF_OK) {
+            best.single_pass = ms <= best_ms ? 0 : 1;
+        }
+    }
+    tune[b].single_pass = best.single_pass;
     if (s == PF_OK && d_tt) {
         const int tmax = expand_tt_max_bpc(b);
         double best_t = 1e30;

@@ -156,11 +156,27 @@ def test_autotune_keeps_counts_and_round_trips(pg, oracle, tmp_path):
         eng.close()
     text = open(path).read()
     assert text.startswith("perft_gpu_amd tuning 1") and "backend ks persistent_variant" in text
+    ks = [l for l in text.splitlines() if l.startswith("backend ks ")][0]
+    assert ks.endswith("single_pass stream") or ks.endswith("single_pass persistent"), ks
     eng = pg.Engine(backend="ks", arena_bytes=2 << 30, tune_file=path)
     try:
         assert np.array_equal(eng.perft_batch(pos, 4, mode=pg.MODE_PERSISTENT), want)
     finally:
         eng.close()
+    old = tmp_path / "old.txt"
+    old.write_text("perft_gpu_amd tuning 1\ncus 256\nbackend ks persistent_variant 0 persistent_blocks_per_cu 3 "
+                   "tt_blocks_per_cu 3\n")
+    eng = pg.Engine(backend="ks", arena_bytes=2 << 30, tune_file=str(old))
+    try:
+        assert np.array_equal(eng.perft_batch(pos, 4), want)
+    finally:
+        eng.close()
+    wrong = tmp_path / "wrong.txt"
+    wrong.write_text("perft_gpu_amd tuning 1\nbackend ks persistent_variant 0 persistent_blocks_per_cu 3 "
+                     "tt_blocks_per_cu 3 single_pass sideways\n")
+    with pytest.raises(pg.PerftError) as ei:
+        pg.Engine(backend="ks", arena_bytes=1 << 30, tune_file=str(wrong))
+    assert ei.value.status == pg.PF_EINVAL
     bad = tmp_path / "bad.txt"
     bad.write_text("not a tuning file\n")
     with pytest.raises(pg.PerftError) as ei:
