@@ -29,6 +29,18 @@ namespace pf {
 
 constexpr int kWG = 256;
 
+// The byte-lookup backend fills nearly all of a CU's LDS with its tables, so a
+// CU holds a single one of its workgroups.  That workgroup is larger (12
+// waves, three per SIMD) so that other waves hide each wave's table-lookup
+// latency; the other backends run several 256-thread workgroups per CU.
+// Measured on C2 (256 positions, perft 5): 256 threads 3.7e11 leaves/s,
+// 512 5.4e11, 768 6.2e11, 1024 5.9e11 (all with the 64 x 1920 tile below).
+constexpr int kByteWG = 768;
+template <int B>
+struct WgOf {
+    static constexpr int v = (B == kByte) ? kByteWG : kWG;
+};
+
 __device__ __forceinline__ u32 wave_incl_scan(u32 v, int lane) {
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -90,21 +102,22 @@ struct LdsEmit {
     }
 };
 
-template <int TP, int CAP>
+template <int TP, int CAP, int WG>
 struct TileLds {
     Board b[TP];
     u32 root[TP];
     u32 sum[TP];
     u32 mv[CAP];
-    u32 wtot[kWG / 64];
-    unsigned long long red[kWG / 64];
+    u32 wtot[WG / 64];
+    unsigned long long red[WG / 64];
     u32 tile, base, ok;
 };
 
 // Process every tile of one level (tiles come from a.tile_ctr).
-template <class SL, int TP, int CAP>
-__device__ __forceinline__ void expand_level(const SL& sl, const LevelArgs& a, const int mode, TileLds<TP, CAP>& s) {
-    static_assert(TP <= kWG && (TP & (TP - 1)) == 0, "one parent per thread, power of two");
+template <class SL, int TP, int CAP, int WG>
+__device__ __forceinline__ void expand_level(const SL& sl, const LevelArgs& a, const int mode,
+                                             TileLds<TP, CAP, WG>& s) {
+    static_assert(TP <= WG && (TP & (TP - 1)) == 0, "one parent per thread, power of two");
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const u32 tp = (a.tile_parents && a.tile_parents < (u32)TP) ? a.tile_parents : (u32)TP;
     const u32 ntiles = (a.n_in + tp - 1) / tp;
@@ -130,7 +143,7 @@ __device__ __forceinline__ void expand_level(const SL& sl, const LevelArgs& a, c
         __syncthreads();
         u32 woff = 0, tot = 0;
 #pragma unroll
-        for (int w = 0; w < kWG / 64; w++) {
+        for (int w = 0; w < WG / 64; w++) {
             const u32 t = s.wtot[w];
             woff += (w < wid) ? t : 0;
             tot += t;
@@ -156,7 +169,7 @@ __device__ __forceinline__ void expand_level(const SL& sl, const LevelArgs& a, c
             __syncthreads();
             const bool wr = (mode == kWrite) && s.ok;
             const u32 base = s.base;
-            for (u32 j = tid; j < m; j += kWG) {
+            for (u32 j = tid; j < m; j += WG) {
                 const u32 e = s.mv[j];
                 const u32 par = (e >> 16) & (TP - 1);
                 const Board c = make_move(s.b[par], e & 0xFFFFu);
@@ -180,7 +193,7 @@ __device__ __forceinline__ void expand_level(const SL& sl, const LevelArgs& a, c
             if (tid == 0) {
                 unsigned long long t = 0;
 #pragma unroll
-                for (int w = 0; w < kWG / 64; w++) t += s.red[w];
+                for (int w = 0; w < WG / 64; w++) t += s.red[w];
                 if (t && a.total) atomicAdd(a.total, t);
                 if (t && uni && a.root_sums) atomicAdd(&a.root_sums[r0], t);
             }
@@ -196,12 +209,17 @@ struct Cfg {
     static constexpr int CAP = 8192;
     static constexpr size_t dyn = 0;
 };
+// Byte backend tile: 64 parents and a 1920-entry move window, the largest
+// window that fits beside the staged tables, so that a tile's moves (about 30
+// per parent) mostly fit one window and each parent is generated once.
 template <>
 struct Cfg<kByte> {
     static constexpr int TP = 64;
-    static constexpr int CAP = 1024;
+    static constexpr int CAP = 1920;
     static constexpr size_t dyn = kByteLds;
 };
+static_assert(sizeof(TileLds<Cfg<kByte>::TP, Cfg<kByte>::CAP, kByteWG>) + kByteLds + 128 <= 160 * 1024,
+              "byte backend: tables plus tile state must fit one CU's LDS");
 
 // Tile shapes of the persistent kernel that the tuner chooses between:
 // variant 0 = 256 parents x 8192-entry move window, 1 = 128 x 4096,
@@ -218,9 +236,9 @@ struct PCfg<kByte, V> {
 };
 
 template <int B, int MODE>
-__global__ __launch_bounds__(kWG) void k_expand(const SliderTables* tabs, LevelArgs a) {
+__global__ __launch_bounds__(WgOf<B>::v) void k_expand(const SliderTables* tabs, LevelArgs a) {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    __shared__ TileLds<Cfg<B>::TP, Cfg<B>::CAP> s;
+    __shared__ TileLds<Cfg<B>::TP, Cfg<B>::CAP, WgOf<B>::v> s;
     const auto sl = stage_backend<B>(tabs, dyn_lds);
     expand_level(sl, a, MODE, s);
 }
@@ -278,16 +296,16 @@ __global__ __launch_bounds__(kWG) void k_range_count(const Board* b, u32 n, unsi
 // All branch decisions between barriers read the same device words, so every
 // workgroup takes the same path and reaches every barrier.
 template <int B, int V>
-__global__ __launch_bounds__(kWG) void k_persistent(const SliderTables* tabs, PersistArgs p) {
+__global__ __launch_bounds__(WgOf<B>::v) void k_persistent(const SliderTables* tabs, PersistArgs p) {
     namespace cg = cooperative_groups;
     cg::grid_group grid = cg::this_grid();
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    __shared__ TileLds<PCfg<B, V>::TP, PCfg<B, V>::CAP> s;
+    __shared__ TileLds<PCfg<B, V>::TP, PCfg<B, V>::CAP, WgOf<B>::v> s;
     const auto sl = stage_backend<B>(tabs, dyn_lds);
     PersistCtl* ctl = p.ctl;
 
     // level 0: root move counts
-    for (u32 i = blockIdx.x * kWG + threadIdx.x; i < p.n_roots; i += gridDim.x * kWG) {
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n_roots; i += gridDim.x * blockDim.x) {
         const Board b = p.roots[i];
         const u32 c = count_moves(b, sl);
         p.roots[i] = with_count(b, c);
@@ -347,10 +365,10 @@ __global__ __launch_bounds__(kWG) void k_persistent(const SliderTables* tabs, Pe
 // Every workgroup of a level launch derives the same placement from words
 // the previous launch finished writing.
 template <int B>
-__global__ __launch_bounds__(kWG) void k_stream_roots(const SliderTables* tabs, PersistArgs p) {
+__global__ __launch_bounds__(WgOf<B>::v) void k_stream_roots(const SliderTables* tabs, PersistArgs p) {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     const auto sl = stage_backend<B>(tabs, dyn_lds);
-    const u32 i = blockIdx.x * kWG + threadIdx.x;
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long c = 0;
     if (i < p.n_roots) {
         const Board b = p.roots[i];
@@ -363,9 +381,9 @@ __global__ __launch_bounds__(kWG) void k_stream_roots(const SliderTables* tabs, 
 }
 
 template <int B, int MODE>
-__global__ __launch_bounds__(kWG) void k_stream_level(const SliderTables* tabs, PersistArgs p, int L) {
+__global__ __launch_bounds__(WgOf<B>::v) void k_stream_level(const SliderTables* tabs, PersistArgs p, int L) {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    __shared__ TileLds<PCfg<B, 0>::TP, PCfg<B, 0>::CAP> s;
+    __shared__ TileLds<PCfg<B, 0>::TP, PCfg<B, 0>::CAP, WgOf<B>::v> s;
     PersistCtl* ctl = p.ctl;
     // an earlier level did not fit: nothing below it is counted (one read per
     // workgroup, so that the whole workgroup leaves together)
@@ -449,11 +467,12 @@ static void* pick_persistent(int backend, int variant) {
 }
 
 static size_t dyn_bytes(int backend) { return backend == kByte ? Cfg<kByte>::dyn : 0; }
+static int wg_size(int backend) { return backend == kByte ? kByteWG : kWG; }
 
 static int blocks_per_cu(void* f, int backend) {
     if (backend == kByte) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_bytes(backend));
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kWG, dyn_bytes(backend)) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, wg_size(backend), dyn_bytes(backend)) != hipSuccess || per_cu < 1)
         per_cu = 1;
     return per_cu;
 }
@@ -480,7 +499,7 @@ hipError_t launch_expand(int backend, int mode, const DevTables& dt, const Level
     LevelArgs args = a;
     const SliderTables* t = dt.t;
     void* kargs[] = {(void*)&t, (void*)&args};
-    return hipLaunchKernel(f, dim3(grid), dim3(kWG), kargs, dyn_bytes(backend), s);
+    return hipLaunchKernel(f, dim3(grid), dim3(wg_size(backend)), kargs, dyn_bytes(backend), s);
 }
 
 hipError_t launch_persistent(int backend, int variant, const DevTables& dt, const PersistArgs& a, int grid,
@@ -489,7 +508,7 @@ hipError_t launch_persistent(int backend, int variant, const DevTables& dt, cons
     PersistArgs args = a;
     const SliderTables* t = dt.t;
     void* kargs[] = {(void*)&t, (void*)&args};
-    return hipLaunchCooperativeKernel(f, dim3(grid), dim3(kWG), kargs, (unsigned)dyn_bytes(backend), s);
+    return hipLaunchCooperativeKernel(f, dim3(grid), dim3(wg_size(backend)), kargs, (unsigned)dyn_bytes(backend), s);
 }
 
 template <int B>
@@ -519,17 +538,18 @@ hipError_t launch_stream(int backend, const DevTables& dt, const PersistArgs& a,
     PersistArgs args = a;
     const SliderTables* t = dt.t;
     const unsigned dyn = (unsigned)dyn_bytes(backend);
+    const int wg = wg_size(backend);
     if (backend == kByte) hipFuncSetAttribute(pick_stream_roots(backend), hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
     {
         void* kargs[] = {(void*)&t, (void*)&args};
-        const hipError_t e = hipLaunchKernel(pick_stream_roots(backend), dim3((a.n_roots + kWG - 1) / kWG), dim3(kWG),
+        const hipError_t e = hipLaunchKernel(pick_stream_roots(backend), dim3((a.n_roots + wg - 1) / wg), dim3(wg),
                                              kargs, dyn, s);
         if (e != hipSuccess) return e;
     }
     for (int L = 0; L <= a.depth - 2; L++) {
         const int mode = (L == a.depth - 2) ? kLeaf : kWrite;
         void* kargs[] = {(void*)&t, (void*)&args, (void*)&L};
-        const hipError_t e = hipLaunchKernel(pick_stream_level(backend, mode), dim3(grid[mode]), dim3(kWG), kargs, dyn, s);
+        const hipError_t e = hipLaunchKernel(pick_stream_level(backend, mode), dim3(grid[mode]), dim3(wg), kargs, dyn, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
