@@ -31,8 +31,12 @@ Also reported on the same box, outside the timed region:
                 the counting work at the vector peak (tt_roofline)
   c5            the 4096 high-mobility positions (--c5-positions), perft 6,
                 128 GiB TT
+  c1            the initial position, perft 5, through the plain path on the
+                GPU (median of 20 runs) and through the oracle's Kogge-Stone
+                perft on one host thread (median of 50 runs)
   cpu_baseline  the oracle (CPU restatement of the reference perft, fancy
-                magics) on the host cores, same C2 positions, same depth
+                magics) on the host cores, same C2 positions, same depth;
+                single_thread: the same on one thread
 """
 import argparse
 import json
@@ -82,7 +86,30 @@ def cpu_baseline(positions, depth, gpu_counts, budget_s=20.0):
     return {"value": nodes / el, "unit": "nodes/s", "cores": threads, "kind": "port",
             "sample": "first %d of the %d C2 positions, perft(%d) each, %d passes, oracle fancy magics, "
                       "%d threads, %.2f s" % (n, len(positions), depth, reps, threads, el),
-            "counts_match_gpu": ok}
+            "counts_match_gpu": ok,
+            "single_thread": {"value": est_leaf_rate, "unit": "nodes/s", "cores": 1,
+                              "sample": "first 4 C2 positions, perft(%d) each, one pass" % (depth - 1)}}
+
+
+def cpu_c1(fen, depth, want, reps=50):
+    """C1 on the CPU as the configuration names it: the oracle's Kogge-Stone
+    perft of one position on one thread, `reps` runs, median time."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    orc = Oracle()
+    old = orc.set_backend(0)  # Kogge-Stone, for the calling thread
+    times, ok = [], True
+    try:
+        for _ in range(reps):
+            t = time.perf_counter()
+            got = orc.perft_fen(fen, depth)
+            times.append(time.perf_counter() - t)
+            ok = ok and got == want
+    finally:
+        orc.set_backend(old)
+    med = sorted(times)[len(times) // 2]
+    return {"value": want / med, "unit": "nodes/s", "median_s": med, "runs": reps, "cores": 1, "kind": "port",
+            "backend": "ks", "counts_match": ok}
 
 
 def run_serving(pg, W, seconds, backend, paced=True):
@@ -347,6 +374,19 @@ def main():
                          "counts_match_batched": bool((plain == counts).all())}
         line["batched_over_plain"] = value / (leaves / tp)
         log("plain: %.3e nodes/s  ratio %.2f" % (leaves / tp, line["batched_over_plain"]))
+        # C1: one position (the initial position) at depth 5 through pf_perft
+        eng.set_backend(W.C1["backend"])
+        c1pos = pg.fen_to_pos(W.C1["fen"])
+        c1t = []
+        for _ in range(21):
+            t = time.perf_counter()
+            c1n = eng.perft(c1pos, W.C1["depth"])
+            c1t.append(time.perf_counter() - t)
+        c1med = sorted(c1t[1:])[len(c1t[1:]) // 2]
+        line["c1"] = {"gpu": {"value": c1n / c1med, "unit": "nodes/s", "median_s": c1med, "runs": 20,
+                              "nodes": c1n, "counts_match": c1n == 4865609}}
+        eng.set_backend(args.backend)
+        log("c1: %s" % line["c1"])
         be = {}
         for b in ("ks", "plain", "fancy", "byte"):
             eng.set_backend(b)
@@ -418,6 +458,10 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(pos, depth, counts)
         log("cpu: %s" % line["cpu_baseline"])
+        if "c1" in line:
+            line["c1"]["cpu"] = cpu_c1(W.C1["fen"], W.C1["depth"], 4865609)
+            line["c1"]["gpu_over_cpu"] = line["c1"]["gpu"]["value"] / line["c1"]["cpu"]["value"]
+            log("c1 cpu: %s" % line["c1"]["cpu"])
     print(json.dumps(line), flush=True)
 
 

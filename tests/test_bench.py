@@ -41,3 +41,13 @@ def test_op_model_per_made_node_matches_the_fixture():
     made = sum(sum(c2["perft"][str(d)]) for d in range(1, 5))
     assert W.C2_MADE_NODES == made
     assert abs(W.C2_OPS_PER_MADE_NODE - c2["op_model"]["int32_lane_ops"] / made) < 1e-9
+
+
+def test_cpu_c1_reports_the_initial_position_count():
+    """the C1 CPU leg: Kogge-Stone oracle on one thread, median of its runs"""
+    sys.path.insert(0, ROOT)
+    import bench
+    from perft_gpu_amd import START_FEN
+    r = bench.cpu_c1(START_FEN, 4, 197281, reps=3)
+    assert r["counts_match"] and r["cores"] == 1 and r["backend"] == "ks" and r["runs"] == 3
+    assert abs(r["value"] - 197281 / r["median_s"]) < 1e-6 * r["value"]
