@@ -44,6 +44,27 @@ __device__ __forceinline__ unsigned long long t_wave_sum64(unsigned long long v)
     return v;
 }
 
+// Adds each lane's v to its *p with one global atomic per run of consecutive
+// lanes that hold the same address.  A level's children are written in their
+// parents' order, so the 64 items of an upsweep wave usually belong to two or
+// three parents; the run's head lane adds the run's total.  Every lane of the
+// wave must call it; a lane with p == nullptr adds nothing.
+__device__ __forceinline__ void wave_run_add(unsigned long long* p, unsigned long long v) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long a = reinterpret_cast<unsigned long long>(p);
+    const unsigned long long prev = __shfl_up(a, 1, 64);
+    const bool head = lane == 0 || prev != a;
+    const unsigned long long heads = __ballot(head);
+    const u32 rid = (u32)__popcll(heads & ((2ULL << lane) - 1));  // runs that start at or below this lane
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {  // suffix sums within each run
+        const unsigned long long ov = __shfl_down(v, d, 64);
+        const u32 orid = __shfl_down(rid, d, 64);
+        if (lane + d < 64 && orid == rid) v += ov;
+    }
+    if (head && p && v) atomicAdd(p, v);
+}
+
 __device__ __forceinline__ bool tt_probe(const uint4* tt, u64 mask, u64 k1, u64 k2, int depth, u64* val) {
     const uint4* bkt = tt + 2 * (k1 & mask);
 #pragma unroll
@@ -429,28 +450,34 @@ __global__ __launch_bounds__(256) void k_roots_tt(const SliderTables* tabs, Boar
 // Item k of a finished level's upsweep (calcPerftNFromPerftNminus1_hash128b,
 // perft_bb.h:854-961): k < n is node k, which adds its exact count to its
 // parent and is stored in the table; n <= k < n + ndup is duplicate k - n,
-// which adds the count of the node that claimed its slot.  Returns 1 for a
-// store.
-__device__ __forceinline__ u32 upsweep_item(const UpSeg& u, u32 k, uint4* tt, u64 mask) {
+// which adds the count of the node that claimed its slot.  The addition is
+// returned in (*dst, *v) for wave_run_add.  Returns 1 for a store.
+__device__ __forceinline__ u32 upsweep_item(const UpSeg& u, u32 k, uint4* tt, u64 mask, unsigned long long** dst,
+                                            unsigned long long* v) {
     if (k < u.n) {
-        const unsigned long long v = u.acc[k];
-        if (u.parent_acc) atomicAdd(&u.parent_acc[u.parent[k]], v);
+        *v = u.acc[k];
+        *dst = u.parent_acc ? &u.parent_acc[u.parent[k]] : nullptr;
         if (tt && u.depth >= kMinTTDepth) {
-            tt_store(tt, mask, u.k1[k], u.k2[k], u.depth, v);
+            tt_store(tt, mask, u.k1[k], u.k2[k], u.depth, *v);
             return 1;
         }
         return 0;
     }
     const u32 d = k - u.n;
-    const u32 c = (u32)u.dup[u.dup_slot[d]].val;
-    if (u.parent_acc) atomicAdd(&u.parent_acc[u.dup_par[d]], u.acc[c]);
+    if (u.parent_acc) {
+        const u32 c = (u32)u.dup[u.dup_slot[d]].val;
+        *v = u.acc[c];
+        *dst = &u.parent_acc[u.dup_par[d]];
+    }
     return 0;
 }
 
 __global__ __launch_bounds__(256) void k_upsweep(UpSeg u, uint4* tt, u64 mask, unsigned long long* stores) {
     const u32 i = blockIdx.x * 256 + threadIdx.x;
-    unsigned long long st = 0;
-    if (i < u.n + u.ndup) st = upsweep_item(u, i, tt, mask);
+    unsigned long long st = 0, v = 0;
+    unsigned long long* dst = nullptr;
+    if (i < u.n + u.ndup) st = upsweep_item(u, i, tt, mask, &dst, &v);
+    wave_run_add(dst, v);
     if (stores) {
         st = t_wave_sum64(st);
         if ((threadIdx.x & 63) == 0 && st) atomicAdd(stores, st);
@@ -501,13 +528,20 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(kTTWaves))
 __global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 mask, int round) {
     const u32 s0 = t->up_round[round], s1 = t->up_round[round + 1];
     const u32 first = t->up_prefix[s0], last = t->up_prefix[s1];
-    for (u32 i = first + blockIdx.x * 256 + threadIdx.x; i < last; i += gridDim.x * 256) {
-        u32 lo = s0, hi = s1;  // largest segment with up_prefix[seg] <= i
-        while (hi - lo > 1) {
-            const u32 mid = (lo + hi) >> 1;
-            if (t->up_prefix[mid] <= i) lo = mid; else hi = mid;
+    // the loop is uniform per wave (wave_run_add needs every lane)
+    for (u32 w = first + blockIdx.x * 256 + (threadIdx.x & ~63u); w < last; w += gridDim.x * 256) {
+        const u32 i = w + (threadIdx.x & 63);
+        unsigned long long v = 0;
+        unsigned long long* dst = nullptr;
+        if (i < last) {
+            u32 lo = s0, hi = s1;  // largest segment with up_prefix[seg] <= i
+            while (hi - lo > 1) {
+                const u32 mid = (lo + hi) >> 1;
+                if (t->up_prefix[mid] <= i) lo = mid; else hi = mid;
+            }
+            upsweep_item(t->up[lo], i - t->up_prefix[lo], tt, mask, &dst, &v);
         }
-        upsweep_item(t->up[lo], i - t->up_prefix[lo], tt, mask);
+        wave_run_add(dst, v);
     }
 }
 
