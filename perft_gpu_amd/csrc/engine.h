@@ -61,7 +61,8 @@ struct Engine {
     ZTables h_z;
     ZTables* d_z = nullptr;
     unsigned long long* d_ttstats = nullptr;  // probes, hits, stores, -, duplicates' moves, duplicates
-    int tt_grid[4] = {};
+    int tt_grid[4] = {};       // TT write levels (tuned workgroups per CU x CUs)
+    int tt_leaf_grid[4] = {};  // TT leaf levels (the leaf kernel's full occupancy)
     bool dedup = true;  // in-level duplicate detection on TT runs
 
     struct TLevel {  // one breadth-first level with TT bookkeeping (80 B/node)

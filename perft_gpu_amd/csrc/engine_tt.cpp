@@ -110,7 +110,7 @@ pf_status Engine::expand_tt(int mode, const TLevel& lv, u64 off, u64 m, TLevel* 
         a.dup_slot = nx->dslot;
     }
     if (timing) PF_HIP(hipEventRecord(ev0, stream));
-    PF_HIP(launch_expand_tt(backend, d_tab, d_z, a, tt_grid[backend], stream));
+    PF_HIP(launch_expand_tt(backend, d_tab, d_z, a, mode == kLeaf ? tt_leaf_grid[backend] : tt_grid[backend], stream));
     if (timing) PF_HIP(hipEventRecord(ev1, stream));
     st.kernel_launches++;
     st.levels_run++;

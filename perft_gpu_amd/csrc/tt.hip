@@ -28,6 +28,10 @@ constexpr int kTCAP = 4096; // LDS move-list window
 // the time: ask for at least this many resident waves per SIMD (register
 // budget 512 / waves) so that more of that latency is hidden.
 constexpr int kTTWaves = 4;
+// The level driver's leaf kernel is built for five waves per SIMD (it spills
+// 40 B per lane for Kogge-Stone): C3 perft 8 0.0879 -> 0.0868 s and C5
+// 2.189 -> 2.162 s against four waves without spills.
+constexpr int kTTLeafWaves = 5;
 
 __device__ __forceinline__ u32 t_wave_incl_scan(u32 v, int lane) {
 #pragma unroll
@@ -414,10 +418,14 @@ __device__ void expand_tt_level(const SL& sl, const TTArgs& a, const ZTables* zt
     }
 }
 
-template <int B>
-__global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(kTTWaves))) void k_expand_tt(const SliderTables* tabs, const ZTables* zt, TTArgs a) {
+// One kernel per mode: with the mode known at compile time the leaf kernel
+// carries none of the write pass, and is built for kTTLeafWaves waves per SIMD.
+template <int B, int M>
+__global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(M == kLeaf ? kTTLeafWaves : kTTWaves)))
+void k_expand_tt(const SliderTables* tabs, const ZTables* zt, TTArgs a) {
     __shared__ TTLds s;
     const auto sl = SlOfT<B>::make(tabs);
+    a.mode = M;
     expand_tt_level(sl, a, zt, s);
 }
 
@@ -613,20 +621,23 @@ hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt,
 }
 
 // ---- launchers ---------------------------------------------------------------
-static void* pick(int backend) {
+template <int M>
+static void* pick_mode(int backend) {
     switch (backend) {
-    case kKS: return reinterpret_cast<void*>(&k_expand_tt<kKS>);
-    case kPlain: return reinterpret_cast<void*>(&k_expand_tt<kPlain>);
-    default: return reinterpret_cast<void*>(&k_expand_tt<kFancy>);  // byte: global tables here
+    case kKS: return reinterpret_cast<void*>(&k_expand_tt<kKS, M>);
+    case kPlain: return reinterpret_cast<void*>(&k_expand_tt<kPlain, M>);
+    default: return reinterpret_cast<void*>(&k_expand_tt<kFancy, M>);  // byte: global tables here
     }
 }
+static void* pick(int backend, int mode) { return mode == kLeaf ? pick_mode<kLeaf>(backend) : pick_mode<kWrite>(backend); }
 
-int expand_tt_max_bpc(int backend) {
+static int tt_bpc(void* f) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pick(backend), kTWG, 0) != hipSuccess || per_cu < 1)
-        per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kTWG, 0) != hipSuccess || per_cu < 1) per_cu = 1;
     return per_cu;
 }
+int expand_tt_max_bpc(int backend) { return tt_bpc(pick(backend, kWrite)); }
+int expand_tt_leaf_bpc(int backend) { return tt_bpc(pick(backend, kLeaf)); }
 
 hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables* zt, const TTArgs& a, int grid,
                             hipStream_t s) {
@@ -635,7 +646,7 @@ hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables
     if ((u32)grid > tiles) grid = (int)tiles;
     TTArgs args = a;
     void* kargs[] = {(void*)&tabs, (void*)&zt, (void*)&args};
-    return hipLaunchKernel(pick(backend), dim3(grid), dim3(kTWG), kargs, 0, s);
+    return hipLaunchKernel(pick(backend, a.mode), dim3(grid), dim3(kTWG), kargs, 0, s);
 }
 
 hipError_t launch_roots_tt(int backend, const SliderTables* tabs, Board* roots, const u64* k1, const u64* k2, u32 n,

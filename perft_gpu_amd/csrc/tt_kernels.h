@@ -137,7 +137,8 @@ struct StepTable {
 int step_grid(int backend, int n_cu);
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
                        int grid, int up_grid, hipStream_t s);
-int expand_tt_max_bpc(int backend);  // most resident workgroups per CU of the TT expand kernel
+int expand_tt_max_bpc(int backend);   // most resident workgroups per CU of the TT write-level kernel
+int expand_tt_leaf_bpc(int backend);  // the same for the TT leaf-level kernel
 hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables* zt, const TTArgs& a, int grid,
                             hipStream_t s);
 // Roots: probe (counted in stats[0], hits in stats[1]) or count their moves.

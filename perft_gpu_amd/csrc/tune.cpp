@@ -36,6 +36,7 @@ void Engine::apply_tuning() {
             const int tmax = expand_tt_max_bpc(b);
             const int tb = (t.tt_bpc > 0 && t.tt_bpc < tmax) ? t.tt_bpc : tmax;
             tt_grid[b] = tb * n_cu;
+            tt_leaf_grid[b] = expand_tt_leaf_bpc(b) * n_cu;
         }
     }
 }
