@@ -52,7 +52,7 @@ PF_HD u64 between_on(u64 line, int a, int b) {  // squares of `line` strictly be
 // king move can reach.
 PF_HD void pins_checks(const Board& b, PosInfo& pi, u64* xray) {
     const u64 pawns = b.pawns & kRanks2to7;
-    const u64 occ = pawns | b.knights | b.bishops | b.rooks | b.kings;
+    const u64 occ = or3(or3(pawns, b.knights, b.bishops), b.rooks, b.kings);
     const u64 us = b.us, them = occ & ~us;
     const u64 K = b.kings & us;
     const int ksq = ctz(K);
@@ -61,7 +61,7 @@ PF_HD void pins_checks(const Board& b, PosInfo& pi, u64* xray) {
     u64 pinned = 0, checkers = 0, block = 0, beyond = 0;
     const u64 kf = file_of(ksq), kr = rank_of(ksq), kd = diag_of(ksq), ka = anti_of(ksq);
     const u64 kz = xray ? king_fill(K) : 0;
-    u64 sn = ((kf | kr) & eR) | ((kd | ka) & eB);
+    u64 sn = either_and(kf, kr, eR) | either_and(kd, ka, eB);
     while (sn) {
         const int s = ctz(sn);
         sn &= sn - 1;
@@ -77,7 +77,7 @@ PF_HD void pins_checks(const Board& b, PosInfo& pi, u64* xray) {
             pinned |= in & us;
         }
     }
-    checkers |= (knight_fill(K) & eN) | ((((K << 7) & kNotH) | ((K << 9) & kNotA)) & eP);
+    checkers |= and_or(knight_fill(K), eN, (((K << 7) & kNotH) | ((K << 9) & kNotA)) & eP);
     if (xray) *xray = beyond;
     pi.occ = occ; pi.us = us; pi.them = them; pi.pawns = pawns; pi.K = K;
     pi.att = 0; pi.pinned = pinned; pi.checkers = checkers; pi.block = block; pi.ksq = ksq;
@@ -86,7 +86,7 @@ PF_HD void pins_checks(const Board& b, PosInfo& pi, u64* xray) {
 // Squares attacked by the opponent's pawns, knights and king.
 PF_HD u64 leaper_threats(const Board& b, const PosInfo& pi) {
     const u64 eP = pi.pawns & pi.them;
-    return ((eP >> 7) & kNotA) | ((eP >> 9) & kNotH) | knight_fill(b.knights & pi.them) | king_fill(b.kings & pi.them);
+    return or3(((eP >> 7) & kNotA) | ((eP >> 9) & kNotH), knight_fill(b.knights & pi.them), king_fill(b.kings & pi.them));
 }
 
 template <class SL>
@@ -98,11 +98,13 @@ PF_HD void analyse(const Board& b, const SL& sl, PosInfo& pi) {
 
 // Knight moves of every knight in `n` onto `tgt`: the eight jump directions
 // are translations, so two knights never share a target within a direction.
+// Each jump is one shift and one three-input and with its wrap mask.
 PF_HD int knight_count(u64 n, u64 tgt) {
-    const u64 l1 = (n >> 1) & 0x7f7f7f7f7f7f7f7fULL, l2 = (n >> 2) & 0x3f3f3f3f3f3f3f3fULL;
-    const u64 r1 = (n << 1) & 0xfefefefefefefefeULL, r2 = (n << 2) & 0xfcfcfcfcfcfcfcfcULL;
-    return popc((l1 << 16) & tgt) + popc((r1 << 16) & tgt) + popc((l1 >> 16) & tgt) + popc((r1 >> 16) & tgt) +
-           popc((l2 << 8) & tgt) + popc((r2 << 8) & tgt) + popc((l2 >> 8) & tgt) + popc((r2 >> 8) & tgt);
+    constexpr u64 kNotAB = 0xfcfcfcfcfcfcfcfcULL, kNotGH = 0x3f3f3f3f3f3f3f3fULL;
+    return popc(and3(shl<17>(n), kNotA, tgt)) + popc(and3(shl<15>(n), kNotH, tgt)) +
+           popc(and3(shl<10>(n), kNotAB, tgt)) + popc(and3(shl<6>(n), kNotGH, tgt)) +
+           popc(and3(shr<15>(n), kNotA, tgt)) + popc(and3(shr<17>(n), kNotH, tgt)) +
+           popc(and3(shr<6>(n), kNotAB, tgt)) + popc(and3(shr<10>(n), kNotGH, tgt));
 }
 
 // Slider moves onto `tgt` for bishops-like set B and rook-like set R, with
@@ -164,7 +166,7 @@ PF_HD CountMasks count_masks(const PosInfo& pi) {
 
 // King moves and castling, given the threat map `att`.
 PF_HD int count_king(const Board& b, const PosInfo& pi, const CountMasks& cm, u64 att) {
-    int n = popc(king_fill(pi.K) & ~pi.us & ~att);
+    int n = popc(andn2(king_fill(pi.K), pi.us, att));
     // castling (MoveGeneratorBitboard.h:3180-3198), never out of check
     const int r = st_rights(b);
     n += (!cm.check && (r & 1) && !(pi.occ & kF1G1) && !(att & kF1G1)) ? 1 : 0;
@@ -189,10 +191,10 @@ PF_HD u32 count_moves_ks(const Board& b) {
     const u64 eB = b.bishops & them, eR = b.rooks & them;
     // our sliders: pinned ones only along their pin line, none while in check
     const u64 frozen = pi.pinned & ~cm.pinMov, free = ~cm.pinMov;
-    const u64 oB = b.bishops & us & ~frozen, oR = b.rooks & us & ~frozen;
+    const u64 oB = and_andn(b.bishops, us, frozen), oR = and_andn(b.rooks, us, frozen);
     const int ksq = pi.ksq;
-    const u64 rv = oR & (free | file_of(ksq)), rh = oR & (free | rank_of(ksq));
-    const u64 bd = oB & (free | diag_of(ksq)), ba = oB & (free | anti_of(ksq));
+    const u64 rv = either_and(free, file_of(ksq), oR), rh = either_and(free, rank_of(ksq), oR);
+    const u64 bd = either_and(free, diag_of(ksq), oB), ba = either_and(free, anti_of(ksq), oB);
     int n = 0;
     // the final step's wrap mask folds into the destination masks
     const u64 tgtA = tgt & kNotA, tgtH = tgt & kNotH;
@@ -254,7 +256,7 @@ PF_HD u32 count_moves(const Board& b, const SL& sl) {
 PF_HD int count_pawns_knights(const Board& b, const PosInfo& pi, const CountMasks& cm) {
     const u64 occ = pi.occ, us = pi.us, them = pi.them, empty = ~occ;
     const u64 P = pi.pawns & us;
-    const u64 Pf = P & ~pi.pinned;
+    const u64 Pf = and_andn(pi.pawns, us, pi.pinned);
     const int ksq = pi.ksq;
     const bool check = cm.check;
     const u64 safe = cm.safe, pinMov = cm.pinMov;
@@ -272,7 +274,8 @@ PF_HD int count_pawns_knights(const Board& b, const PosInfo& pi, const CountMask
         n += popc(p1 & safe) + 3 * popc(p1 & safe & kRank8);
         const u64 dmask = check ? ~safe : ~0ULL;
         n += popc((((p1 & dmask) & kRank3) << 8) & empty & safe);
-        const u64 cl = ((capL << 7) & kNotH) & them & safe, cr = ((capR << 9) & kNotA) & them & safe;
+        const u64 ts = them & safe;
+        const u64 cl = and3(shl<7>(capL), kNotH, ts), cr = and3(shl<9>(capR), kNotA, ts);
         n += popc(cl) + popc(cr) + 3 * (popc(cl & kRank8) + popc(cr & kRank8));
     }
     // en passant (MoveGeneratorBitboard.h:3058-3107; in check :2873-2890)
@@ -310,7 +313,7 @@ PF_HD int count_pawns_knights(const Board& b, const PosInfo& pi, const CountMask
         }
     }
     // knights (pinned ones never move)
-    n += knight_count(b.knights & us & ~pi.pinned, cm.tgt);
+    n += knight_count(and_andn(b.knights, us, pi.pinned), cm.tgt);
     return n;
 }
 
@@ -429,14 +432,14 @@ PF_HD void generate(const Board& b, const SL& sl, const PosInfo& pi, Emit& emit)
 PF_HD Board make_move(const Board& b, u32 m) {
     const int from = m & 63, to = (m >> 6) & 63, fl = (m >> 12) & 15;
     const u64 src = bit(from), dst = bit(to), mv = src | dst;
-    u64 P = b.pawns & kRanks2to7 & ~dst;
+    u64 P = and_andn(b.pawns, kRanks2to7, dst);
     u64 N = b.knights & ~dst, B = b.bishops & ~dst, R = b.rooks & ~dst, Kg = b.kings;
-    u64 us = (b.us ^ src) | dst;
-    P ^= mv & (0 - ((P >> from) & 1));
-    N ^= mv & (0 - ((N >> from) & 1));
-    B ^= mv & (0 - ((B >> from) & 1));
-    R ^= mv & (0 - ((R >> from) & 1));
-    Kg ^= mv & (0 - ((Kg >> from) & 1));
+    u64 us = xor_or(b.us, src, dst);
+    P = xor_and(P, mv, 0 - ((P >> from) & 1));
+    N = xor_and(N, mv, 0 - ((N >> from) & 1));
+    B = xor_and(B, mv, 0 - ((B >> from) & 1));
+    R = xor_and(R, mv, 0 - ((R >> from) & 1));
+    Kg = xor_and(Kg, mv, 0 - ((Kg >> from) & 1));
     if (fl & 8) {
         P &= ~dst;
         N |= (fl == kFlPromoN) ? dst : 0;
@@ -455,7 +458,7 @@ PF_HD Board make_move(const Board& b, u32 m) {
         const u64 adj = ((dst << 1) & kNotA) | ((dst >> 1) & kNotH);
         if (adj & P & ~us) ep = (from & 7) + 1;
     }
-    const u64 occ = P | N | B | R | Kg;
+    const u64 occ = or3(or3(P, N, B), R, Kg);
     Board c;
     c.us = flip(occ & ~us);
     c.pawns = flip(P) | (u64)(((rights >> 2) & 3) | ((rights & 3) << 2)) | ((u64)ep << 4);
