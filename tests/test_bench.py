@@ -15,7 +15,7 @@ def test_more_than_one_gpu_is_refused_with_a_message():
     assert "one GPU" in r.stderr and r.stdout == ""
 
 
-def test_tt_roofline_phases_add_up():
+def test_tt_roofline_takes_the_larger_phase():
     sys.path.insert(0, ROOT)
     import bench
     from perft_gpu_amd.workloads import C2_OPS_PER_MADE_NODE
@@ -28,7 +28,9 @@ def test_tt_roofline_phases_add_up():
     t_valu = (10000 - 2000 - 100) * C2_OPS_PER_MADE_NODE / (bench.VALU_PEAK_TOPS * 1e12)
     assert r["random_accesses"] == rnd and r["counted_nodes"] == 7900
     assert abs(r["t_memory_s"] - t_mem) < 1e-15 and abs(r["t_valu_s"] - t_valu) < 1e-15
-    assert abs(r["frac"] - (t_mem + t_valu) / 1e-3) < 1e-12
+    assert abs(r["frac"] - max(t_mem, t_valu) / 1e-3) < 1e-12
+    assert abs(r["serial_frac"] - (t_mem + t_valu) / 1e-3) < 1e-12
+    assert r["bound"] == ("valu" if t_valu >= t_mem else "hbm-random")
     json.dumps(r)
 
 
