@@ -493,6 +493,20 @@ __global__ __launch_bounds__(256) void k_upsweep(UpSeg u, uint4* tt, u64 mask, u
 }
 
 // ---- serving step: many jobs' segments in one launch ---------------------------
+// A segment's arguments, read from LDS, as wave-uniform values: each dword
+// goes through readfirstlane, so the pointers and sizes sit in scalar
+// registers as a kernel argument's would, instead of taking some forty
+// vector registers (and spilling) in every lane.
+__device__ __forceinline__ TTArgs uniform_args(const TTArgs& src) {
+    static_assert(sizeof(TTArgs) % 4 == 0, "TTArgs is copied dword by dword");
+    TTArgs a;
+    const u32* from = reinterpret_cast<const u32*>(&src);
+    u32* to = reinterpret_cast<u32*>(&a);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(TTArgs) / 4); i++) to[i] = __builtin_amdgcn_readfirstlane(from[i]);
+    return a;
+}
+
 // Tiles of all expand segments form one queue (StepTable::tile_prefix); a
 // block copies its segment's arguments to LDS and runs the tile.  Segments of
 // different jobs touch disjoint memory except the lockless table.
@@ -527,7 +541,7 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(kTTWaves))
             cur = seg;
         }
         __syncthreads();
-        tt_tile(sl, sa, g - t->tile_prefix[seg], zt, s);
+        tt_tile(sl, uniform_args(sa), g - t->tile_prefix[seg], zt, s);
     }
 }
 
