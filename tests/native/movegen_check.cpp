@@ -136,4 +136,25 @@ int mg_zkey_children_check(const pf_pos48* p, uint64_t seed) {
     }
     return bad;
 }
+
+// every three-input logic helper (board.h) against the two-input expression it
+// replaces, on random words: pins each truth-table immediate
+int mg_bitop3_check(uint64_t seed, int n) {
+    u64 x = seed ? seed : 1;
+    auto nx = [&x]() { x ^= x << 13; x ^= x >> 7; x ^= x << 17; return x; };
+    int bad = 0;
+    for (int i = 0; i < n; i++) {
+        const u64 a = nx(), b = nx(), c = nx();
+        bad += or_and(a, b, c) != (a | (b & c));
+        bad += or3(a, b, c) != (a | b | c);
+        bad += and3(a, b, c) != (a & b & c);
+        bad += and_or(a, b, c) != ((a & b) | c);
+        bad += either_and(a, b, c) != ((a | b) & c);
+        bad += and_andn(a, b, c) != (a & b & ~c);
+        bad += andn2(a, b, c) != (a & ~b & ~c);
+        bad += xor_and(a, b, c) != (a ^ (b & c));
+        bad += xor_or(a, b, c) != ((a ^ b) | c);
+    }
+    return bad;
+}
 }

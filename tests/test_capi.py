@@ -187,3 +187,10 @@ def test_zobrist_incremental_keys(pg, mg):
     for row in rows:
         p = pg.Pos48(*[int(x) for x in row])
         assert mg.mg_zkey_children_check(ctypes.byref(p), 99) == 0, pg.pos_to_fen(p)
+
+
+def test_three_input_logic_helpers(mg):
+    """board.h's bitop3 helpers (host form: the same truth table the GPU
+    instruction takes) equal the two-input expressions they replace"""
+    mg.mg_bitop3_check.argtypes = [ctypes.c_uint64, ctypes.c_int]
+    assert mg.mg_bitop3_check(20250101, 20000) == 0
