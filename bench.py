@@ -33,7 +33,8 @@ Also reported on the same box, outside the timed region:
   c5            the 4096 high-mobility positions (--c5-positions), perft 6,
                 128 GiB TT
   c1            the initial position, perft 5, through the plain path on the
-                GPU (median of 20 runs) and through the oracle's Kogge-Stone
+                GPU (median of 20 runs; gpu_stream: the same as a one-root
+                stream-ordered batch) and through the oracle's Kogge-Stone
                 perft on one host thread (median of 50 runs)
   cpu_baseline  the oracle (CPU restatement of the reference perft, fancy
                 magics) on the host cores, same C2 positions, same depth;
@@ -389,6 +390,16 @@ def main():
         c1med = sorted(c1t[1:])[len(c1t[1:]) // 2]
         line["c1"] = {"gpu": {"value": c1n / c1med, "unit": "nodes/s", "median_s": c1med, "runs": 20,
                               "nodes": c1n, "counts_match": c1n == 4865609}}
+        # the same position as a one-root batch in the stream-ordered mode:
+        # no host round trip between levels
+        c1s = []
+        for _ in range(21):
+            t = time.perf_counter()
+            c1b = int(eng.perft_batch([c1pos], W.C1["depth"], mode=pg.MODE_STREAM)[0])
+            c1s.append(time.perf_counter() - t)
+        c1smed = sorted(c1s[1:])[len(c1s[1:]) // 2]
+        line["c1"]["gpu_stream"] = {"value": c1b / c1smed, "unit": "nodes/s", "median_s": c1smed, "runs": 20,
+                                    "counts_match": c1b == 4865609}
         eng.set_backend(args.backend)
         log("c1: %s" % line["c1"])
         be = {}
