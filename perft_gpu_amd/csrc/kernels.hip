@@ -235,10 +235,31 @@ struct PCfg<kByte, V> {
     static constexpr int CAP = Cfg<kByte>::CAP;
 };
 
+// Tile shape of the per-level kernels (k_expand, k_stream_level): 256 x 4096
+// (variant 2), whose smaller LDS footprint lets more workgroups share a CU
+// than the 256 x 8192 shape.  The byte backend keeps its own shape.
+template <int B>
+struct LevelV {
+    static constexpr int v = (B == kByte) ? 0 : 2;
+};
+// Waves per SIMD the per-level kernels are built for (1 = no constraint).
+// The leaf kernel is built for five, so five workgroups share a CU; the magic
+// backends' fits 96 VGPRs as it is, Kogge-Stone's spills 20 B per lane.  C2
+// leaves/s of the stream-ordered form, 256 x 8192 at three per CU -> 256 x
+// 4096 at four -> at five: plain 1.05e12 -> 1.18e12 -> 1.30e12, fancy
+// 1.08e12 -> 1.20e12 -> 1.31e12 (their table reads from L2 want the extra
+// waves), Kogge-Stone 1.470e12 -> 1.479e12 -> 1.500e12.  The write kernels
+// would spill more and stay at four.
 template <int B, int MODE>
-__global__ __launch_bounds__(WgOf<B>::v) void k_expand(const SliderTables* tabs, LevelArgs a) {
+struct LevelWaves {
+    static constexpr int v = (B != kByte && MODE == kLeaf) ? 5 : 1;
+};
+
+template <int B, int MODE>
+__global__ __launch_bounds__(WgOf<B>::v) __attribute__((amdgpu_waves_per_eu(LevelWaves<B, MODE>::v)))
+void k_expand(const SliderTables* tabs, LevelArgs a) {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    __shared__ TileLds<Cfg<B>::TP, Cfg<B>::CAP, WgOf<B>::v> s;
+    __shared__ TileLds<PCfg<B, LevelV<B>::v>::TP, PCfg<B, LevelV<B>::v>::CAP, WgOf<B>::v> s;
     const auto sl = stage_backend<B>(tabs, dyn_lds);
     expand_level(sl, a, MODE, s);
 }
@@ -380,31 +401,11 @@ __global__ __launch_bounds__(WgOf<B>::v) void k_stream_roots(const SliderTables*
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(&p.ctl->totals[1], c);
 }
 
-// Tile shape of the stream-ordered level kernel: 256 x 4096 (variant 2), whose
-// smaller LDS footprint lets more workgroups share a CU than the 256 x 8192
-// shape.  The byte backend keeps its own shape.
-template <int B>
-struct SV {
-    static constexpr int v = (B == kByte) ? 0 : 2;
-};
-// Waves per SIMD the stream-ordered level kernel is built for (1 = no
-// constraint).  The leaf kernel is built for five, so five workgroups share a
-// CU; the magic backends' fits 96 VGPRs as it is, Kogge-Stone's spills 28 B
-// per lane.  C2 leaves/s, 256 x 8192 at three per CU -> 256 x 4096 at four ->
-// at five: plain 1.05e12 -> 1.18e12 -> 1.30e12, fancy 1.08e12 -> 1.20e12 ->
-// 1.31e12 (their table reads from L2 want the extra waves), Kogge-Stone
-// 1.470e12 -> 1.479e12 -> 1.500e12.  The write kernels would spill more and
-// stay at four.
 template <int B, int MODE>
-struct StreamWaves {
-    static constexpr int v = (B != kByte && MODE == kLeaf) ? 5 : 1;
-};
-
-template <int B, int MODE>
-__global__ __launch_bounds__(WgOf<B>::v) __attribute__((amdgpu_waves_per_eu(StreamWaves<B, MODE>::v)))
+__global__ __launch_bounds__(WgOf<B>::v) __attribute__((amdgpu_waves_per_eu(LevelWaves<B, MODE>::v)))
 void k_stream_level(const SliderTables* tabs, PersistArgs p, int L) {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    __shared__ TileLds<PCfg<B, SV<B>::v>::TP, PCfg<B, SV<B>::v>::CAP, WgOf<B>::v> s;
+    __shared__ TileLds<PCfg<B, LevelV<B>::v>::TP, PCfg<B, LevelV<B>::v>::CAP, WgOf<B>::v> s;
     PersistCtl* ctl = p.ctl;
     // an earlier level did not fit: nothing below it is counted (one read per
     // workgroup, so that the whole workgroup leaves together)
@@ -429,7 +430,7 @@ void k_stream_level(const SliderTables* tabs, PersistArgs p, int L) {
     a.tile_ctr = &ctl->tile_ctr[L];
     a.err = &ctl->err;
     a.out_count = &ctl->out_count[L];
-    a.tile_parents = level_tile_parents(n, gridDim.x, PCfg<B, SV<B>::v>::TP);
+    a.tile_parents = level_tile_parents(n, gridDim.x, PCfg<B, LevelV<B>::v>::TP);
     if (MODE == kLeaf) {
         a.out = nullptr;
         a.out_root = nullptr;
