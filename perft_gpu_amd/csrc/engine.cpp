@@ -68,7 +68,8 @@ pf_status Engine::init(const pf_config& cfg) {
     PF_HIP(hipMalloc(&d_pctl, sizeof(PersistCtl)));
     for (int b = 0; b < 4; b++)
         for (int m = 0; m < 2; m++) {
-            grid[b][m] = expand_grid(b, m, n_cu);
+            grid[b][m] = expand_grid(b, m, 0, n_cu);
+            grid_small[b][m] = expand_grid(b, m, 1, n_cu);
             sgrid[b][m] = stream_grid(b, m, n_cu);
         }
     apply_tuning();
@@ -147,12 +148,15 @@ pf_status Engine::expand(int mode, const Board* in, const u32* in_root, u64 n_in
     a.total = next_total ? d_total : nullptr;
     a.tile_ctr = d_ctrl + 1;
     a.err = d_ctrl + 0;
-    int g = grid[backend][mode];
+    // a level with fewer than 64 parents per workgroup of the wide launch is
+    // latency-bound: it takes the small-level shape
+    const int small = n_in < (u64)grid[backend][mode] * 64 ? 1 : 0;
+    int g = small ? grid_small[backend][mode] : grid[backend][mode];
     a.tile_parents = level_tile_parents(n_in, (u64)g, 256);
     const u64 tiles = (n_in + a.tile_parents - 1) / a.tile_parents;
     if ((u64)g > tiles) g = (int)tiles;
     if (timing) PF_HIP(hipEventRecord(ev0, stream));
-    PF_HIP(launch_expand(backend, mode, DevTables{d_tab}, a, g, stream));
+    PF_HIP(launch_expand(backend, mode, small, DevTables{d_tab}, a, g, stream));
     if (timing) PF_HIP(hipEventRecord(ev1, stream));
     st.kernel_launches++;
     st.levels_run++;

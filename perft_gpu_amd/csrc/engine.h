@@ -33,7 +33,8 @@ struct Engine {
     u64 sums_cap = 0;
     unsigned long long* h_pinned = nullptr;  // small pinned readback buffer
 
-    int grid[4][2] = {};
+    int grid[4][2] = {};        // level-loop launches on large levels (wide tile, five waves per SIMD)
+    int grid_small[4][2] = {};  // the same on small levels (256 x 8192 tile)
     int pgrid[4] = {};
     int sgrid[4][2] = {};  // stream-ordered level launches: [backend][kLeaf / kWrite]
     // launch parameters per backend, chosen by autotune() or a tune file

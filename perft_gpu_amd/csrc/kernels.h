@@ -38,12 +38,13 @@ PF_HD u32 level_tile_parents(u64 n, u64 grid, u32 tp_max) {
 
 enum ExpandMode : int { kLeaf = 0, kWrite = 1 };
 
-// Occupancy-derived grid size for the expand kernel (workgroups).
-int expand_grid(int backend, int mode, int n_cu);
+// Occupancy-derived grid size for the expand kernel (workgroups); small = 1
+// for the shape the level loop uses on small levels.
+int expand_grid(int backend, int mode, int small, int n_cu);
 
 hipError_t launch_init_roots(int backend, const DevTables& dt, Board* roots, u32 n,
                              unsigned long long* root_sums, hipStream_t s);
-hipError_t launch_expand(int backend, int mode, const DevTables& dt, const LevelArgs& a,
+hipError_t launch_expand(int backend, int mode, int small, const DevTables& dt, const LevelArgs& a,
                          int grid, hipStream_t s);
 hipError_t launch_range_count(const Board* b, u32 n, unsigned long long* out, hipStream_t s);
 // Roots already in device memory (reference layout) -> boards + root ids.

@@ -255,11 +255,15 @@ struct LevelWaves {
     static constexpr int v = (B != kByte && MODE == kLeaf) ? 5 : 1;
 };
 
-template <int B, int MODE>
-__global__ __launch_bounds__(WgOf<B>::v) __attribute__((amdgpu_waves_per_eu(LevelWaves<B, MODE>::v)))
+// The host-driven level loop has both shapes: the wide one above for large
+// levels, and for small ones (few parents per workgroup, latency-bound) the
+// 256 x 8192 tile without a waves-per-SIMD target, which does not spill.
+template <int B, int MODE, int V>
+__global__ __launch_bounds__(WgOf<B>::v)
+__attribute__((amdgpu_waves_per_eu(V == LevelV<B>::v ? LevelWaves<B, MODE>::v : 1)))
 void k_expand(const SliderTables* tabs, LevelArgs a) {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    __shared__ TileLds<PCfg<B, LevelV<B>::v>::TP, PCfg<B, LevelV<B>::v>::CAP, WgOf<B>::v> s;
+    __shared__ TileLds<PCfg<B, V>::TP, PCfg<B, V>::CAP, WgOf<B>::v> s;
     const auto sl = stage_backend<B>(tabs, dyn_lds);
     expand_level(sl, a, MODE, s);
 }
@@ -457,16 +461,17 @@ void k_stream_level(const SliderTables* tabs, PersistArgs p, int L) {
 
 // ---- launch plumbing --------------------------------------------------------
 template <int B, int MODE>
-static void* expand_fn() {
-    return reinterpret_cast<void*>(&k_expand<B, MODE>);
+static void* expand_fn(int small) {
+    if (B == kByte || small) return reinterpret_cast<void*>(&k_expand<B, MODE, 0>);
+    return reinterpret_cast<void*>(&k_expand<B, MODE, LevelV<B>::v>);
 }
 
-static void* pick_expand(int backend, int mode) {
+static void* pick_expand(int backend, int mode, int small) {
     switch (backend) {
-    case kKS: return mode ? expand_fn<kKS, kWrite>() : expand_fn<kKS, kLeaf>();
-    case kPlain: return mode ? expand_fn<kPlain, kWrite>() : expand_fn<kPlain, kLeaf>();
-    case kFancy: return mode ? expand_fn<kFancy, kWrite>() : expand_fn<kFancy, kLeaf>();
-    default: return mode ? expand_fn<kByte, kWrite>() : expand_fn<kByte, kLeaf>();
+    case kKS: return mode ? expand_fn<kKS, kWrite>(small) : expand_fn<kKS, kLeaf>(small);
+    case kPlain: return mode ? expand_fn<kPlain, kWrite>(small) : expand_fn<kPlain, kLeaf>(small);
+    case kFancy: return mode ? expand_fn<kFancy, kWrite>(small) : expand_fn<kFancy, kLeaf>(small);
+    default: return mode ? expand_fn<kByte, kWrite>(small) : expand_fn<kByte, kLeaf>(small);
     }
 }
 
@@ -499,7 +504,9 @@ static int blocks_per_cu(void* f, int backend) {
     return per_cu;
 }
 
-int expand_grid(int backend, int mode, int n_cu) { return blocks_per_cu(pick_expand(backend, mode), backend) * n_cu; }
+int expand_grid(int backend, int mode, int small, int n_cu) {
+    return blocks_per_cu(pick_expand(backend, mode, small), backend) * n_cu;
+}
 int persistent_max_bpc(int backend, int variant) { return blocks_per_cu(pick_persistent(backend, variant), backend); }
 
 hipError_t launch_init_roots(int backend, const DevTables& dt, Board* roots, u32 n,
@@ -514,10 +521,10 @@ hipError_t launch_init_roots(int backend, const DevTables& dt, Board* roots, u32
     return hipGetLastError();
 }
 
-hipError_t launch_expand(int backend, int mode, const DevTables& dt, const LevelArgs& a, int grid,
+hipError_t launch_expand(int backend, int mode, int small, const DevTables& dt, const LevelArgs& a, int grid,
                          hipStream_t s) {
     if (!a.n_in) return hipSuccess;
-    void* f = pick_expand(backend, mode);
+    void* f = pick_expand(backend, mode, small);
     LevelArgs args = a;
     const SliderTables* t = dt.t;
     void* kargs[] = {(void*)&t, (void*)&args};
