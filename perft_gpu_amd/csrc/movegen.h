@@ -196,45 +196,46 @@ PF_HD u32 count_moves_ks(const Board& b) {
     const u64 rv = either_and(free, file_of(ksq), oR), rh = either_and(free, rank_of(ksq), oR);
     const u64 bd = either_and(free, diag_of(ksq), oB), ba = either_and(free, anti_of(ksq), oB);
     int n = 0;
-    // the final step's wrap mask folds into the destination masks
-    const u64 tgtA = tgt & kNotA, tgtH = tgt & kNotH;
-    u64 attA = 0, attH = 0;  // threat rays that still need the kNotA / kNotH mask
-#define PF_KS_DIR(D, C, E, O, ATT, TGT)                 \
-    {                                                   \
-        ATT |= step<D>(ray_reach<D>(E, C));             \
-        n += popc(step<D>(ray_reach<D>(O, C)) & TGT);   \
+    // each direction: the enemy ray (threat map) and our moves' count
+#define PF_KS_DIR(D, C, E, O, RAY)                                          \
+    {                                                                        \
+        RAY = step<D>(ray_reach<D>(E, C));                                   \
+        n += popc(and3(step<D>(ray_reach<D>(O, C)), tgt, step_mask<D>()));   \
     }
+    u64 rN, rS, rE, rW, rNE, rSW, rNW, rSE;
     {
         const RayChain s = ray_chain<-8>(empty), nn = north_chain(s);
-        PF_KS_DIR(8, nn, eR, rv, att, tgt)
-        PF_KS_DIR(-8, s, eR, rv, att, tgt)
+        PF_KS_DIR(8, nn, eR, rv, rN)
+        PF_KS_DIR(-8, s, eR, rv, rS)
     }
     {
         const RayChain c = ray_chain<1>(empty);
-        PF_KS_DIR(1, c, eR, rh, attA, tgtA)
+        PF_KS_DIR(1, c, eR, rh, rE)
     }
     {
         const RayChain c = ray_chain<-1>(empty);
-        PF_KS_DIR(-1, c, eR, rh, attH, tgtH)
+        PF_KS_DIR(-1, c, eR, rh, rW)
     }
     {
         const RayChain c = ray_chain<9>(empty);
-        PF_KS_DIR(9, c, eB, bd, attA, tgtA)
+        PF_KS_DIR(9, c, eB, bd, rNE)
     }
     {
         const RayChain c = ray_chain<-9>(empty);
-        PF_KS_DIR(-9, c, eB, bd, attH, tgtH)
+        PF_KS_DIR(-9, c, eB, bd, rSW)
     }
     {
         const RayChain c = ray_chain<7>(empty);
-        PF_KS_DIR(7, c, eB, ba, attH, tgtH)
+        PF_KS_DIR(7, c, eB, ba, rNW)
     }
     {
         const RayChain c = ray_chain<-7>(empty);
-        PF_KS_DIR(-7, c, eB, ba, attA, tgtA)
+        PF_KS_DIR(-7, c, eB, ba, rSE)
     }
 #undef PF_KS_DIR
-    att = or_and(or_and(att, attA, kNotA), attH, kNotH);
+    // the rays that still need the kNotA / kNotH wrap mask, then the map
+    const u64 attA = or3(rE, rNE, rSE), attH = or3(rW, rSW, rNW);
+    att = or_and(or_and(or3(att, rN, rS), attA, kNotA), attH, kNotH);
     return (u32)(n + count_king(b, pi, cm, att) + count_pawns_knights(b, pi, cm));
 }
 
