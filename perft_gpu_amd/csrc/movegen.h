@@ -269,8 +269,9 @@ PF_HD int count_pawns_knights(const Board& b, const PosInfo& pi, const CountMask
         // a1-h8-pinned ones capture up-right, a8-h1-pinned ones up-left.
         // pinMov is empty in check, and out of check `safe` is every square,
         // so they share the unpinned pawns' masks.
-        const u64 push = Pf | (P & pinMov & file_of(ksq));
-        const u64 capR = Pf | (P & pinMov & diag_of(ksq)), capL = Pf | (P & pinMov & anti_of(ksq));
+        const u64 pm = P & pinMov;
+        const u64 push = or_and(Pf, pm, file_of(ksq));
+        const u64 capR = or_and(Pf, pm, diag_of(ksq)), capL = or_and(Pf, pm, anti_of(ksq));
         const u64 p1 = (push << 8) & empty;
         n += popc(p1 & safe) + 3 * popc(p1 & safe & kRank8);
         const u64 dmask = check ? ~safe : ~0ULL;
