@@ -353,7 +353,7 @@ def main():
                      # HBM bytes per step: FETCH_SIZE x2.00 + WRITE_SIZE x1.00 over the step's kernels
                      # (rocprofv3 --pmc, calibrated on a copy of known size;
                      # profiles/k_stream_level_ks_c2_d5_counters.md)
-                     "traffic": 9.300e8,
+                     "traffic": 9.522e8,
                      "kernel": ("k_stream_roots + %d x k_stream_level<%s>" % (depth - 1, args.backend)
                                 if single == "stream" else "k_persistent<%s>" % args.backend),
                      "kernel_ms": kernel_ms,
