@@ -244,12 +244,12 @@ struct LevelV {
 };
 // Waves per SIMD the per-level kernels are built for (1 = no constraint).
 // The leaf kernel is built for five, so five workgroups share a CU; the magic
-// backends' fits 96 VGPRs as it is, Kogge-Stone's spills 20 B per lane.  C2
-// leaves/s of the stream-ordered form, 256 x 8192 at three per CU -> 256 x
-// 4096 at four -> at five: plain 1.05e12 -> 1.18e12 -> 1.30e12, fancy
-// 1.08e12 -> 1.20e12 -> 1.31e12 (their table reads from L2 want the extra
-// waves), Kogge-Stone 1.470e12 -> 1.479e12 -> 1.500e12.  The write kernels
-// would spill more and stay at four.
+// backends' fits 96 VGPRs as it is, Kogge-Stone's spills 28 B per lane
+// (outside its make+count loop).  C2 leaves/s of the stream-ordered form,
+// 256 x 8192 at three per CU -> 256 x 4096 at four -> at five: plain 1.05e12
+// -> 1.18e12 -> 1.30e12, fancy 1.08e12 -> 1.20e12 -> 1.31e12 (their table
+// reads from L2 want the extra waves), Kogge-Stone 1.470e12 -> 1.479e12 ->
+// 1.500e12.  The write kernels would spill more and stay at four.
 template <int B, int MODE>
 struct LevelWaves {
     static constexpr int v = (B != kByte && MODE == kLeaf) ? 5 : 1;
