@@ -499,11 +499,12 @@ __global__ __launch_bounds__(256) void k_upsweep(UpSeg u, uint4* tt, u64 mask, u
 // vector registers (and spilling) in every lane.
 __device__ __forceinline__ TTArgs uniform_args(const TTArgs& src) {
     static_assert(sizeof(TTArgs) % 4 == 0, "TTArgs is copied dword by dword");
-    TTArgs a;
-    const u32* from = reinterpret_cast<const u32*>(&src);
-    u32* to = reinterpret_cast<u32*>(&a);
+    u32 w[sizeof(TTArgs) / 4];
+    __builtin_memcpy(w, &src, sizeof(w));
 #pragma unroll
-    for (int i = 0; i < (int)(sizeof(TTArgs) / 4); i++) to[i] = __builtin_amdgcn_readfirstlane(from[i]);
+    for (int i = 0; i < (int)(sizeof(TTArgs) / 4); i++) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    TTArgs a;
+    __builtin_memcpy(&a, w, sizeof(a));
     return a;
 }
 
