@@ -1138,11 +1138,17 @@ uint64_t orc_perft(const orc_pos *p, int depth) {
     return perft_rec(p, depth);
 }
 
-typedef struct { const orc_pos *p; int n, depth, stride, start, backend; uint64_t *out; } job_t;
+/* Positions are handed out one at a time from a shared counter, so that
+ * subtrees of very different sizes keep every thread busy to the end. */
+typedef struct { const orc_pos *p; int n, depth, backend; int *next; uint64_t *out; } job_t;
 static void *perft_worker(void *arg) {
     job_t *j = (job_t *)arg;
     g_backend = j->backend;
-    for (int i = j->start; i < j->n; i += j->stride) j->out[i] = orc_perft(&j->p[i], j->depth);
+    for (;;) {
+        const int i = __atomic_fetch_add(j->next, 1, __ATOMIC_RELAXED);
+        if (i >= j->n) break;
+        j->out[i] = orc_perft(&j->p[i], j->depth);
+    }
     return 0;
 }
 
@@ -1151,8 +1157,9 @@ void orc_perft_batch(const orc_pos *p, int n, int depth, int nthreads, uint64_t 
     if (nthreads > 256) nthreads = 256;
     pthread_t th[256];
     job_t jobs[256];
+    int next = 0;
     for (int t = 0; t < nthreads; t++) {
-        jobs[t] = (job_t){p, n, depth, nthreads, t, g_backend, out};
+        jobs[t] = (job_t){p, n, depth, g_backend, &next, out};
         pthread_create(&th[t], 0, perft_worker, &jobs[t]);
     }
     for (int t = 0; t < nthreads; t++) pthread_join(th[t], 0);
