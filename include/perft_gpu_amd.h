@@ -130,6 +130,17 @@ pf_status pf_perft(pf_ctx *ctx, const pf_pos48 *pos, int depth, uint64_t *out);
  * of another device is PF_EINVAL. */
 pf_status pf_perft_batch(pf_ctx *ctx, const pf_pos48 *pos, int n, int depth, uint64_t *out);
 pf_status pf_perft_batch_mode(pf_ctx *ctx, const pf_pos48 *pos, int n, int depth, int mode, uint64_t *out);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The stream-ordered batch (PF_MODE_STREAM) with `pos` and `out`
+ * both in device memory of the context's device (else PF_EINVAL); its control
+ * block, converted roots and frontier levels live in one fixed 512 MiB device
+ * array that the library holds for each device it runs on until the process
+ * ends, resolved on the first call on that device, so the call allocates
+ * nothing.  The counts accumulate in `out`.  PF_ENOMEM when the batch's
+ * frontier exceeds the array (256 positions at perft 5 fit).  Every call on
+ * one GPU uses the same array: calls that overlap on one GPU race. */
+pf_status pf_perft_batch_noalloc(pf_ctx *ctx, const pf_pos48 *pos, int n, int depth, uint64_t *out);
 
 /* Divided perft for the UCI "go perft": per legal root move, the move
  * (from | to << 6 | flag << 12, reference CMove flags without the capture

@@ -68,7 +68,7 @@ _lib = None
 
 # every symbol the C ABI declares (checked by the CPU test suite)
 EXPORTS = ["pf_init", "pf_destroy", "pf_strerror", "pf_fen_to_pos", "pf_pos_to_fen", "pf_perft",
-           "pf_perft_batch", "pf_perft_batch_mode", "pf_divide", "pf_move_to_uci", "pf_apply_uci",
+           "pf_perft_batch", "pf_perft_batch_mode", "pf_perft_batch_noalloc", "pf_divide", "pf_move_to_uci", "pf_apply_uci",
            "pf_set_backend", "pf_set_dedup", "pf_tune", "pf_records", "pf_tt_save", "pf_tt_load", "pf_get_stats", "pf_reset_stats",
            "pf_sync", "pf_probe", "pf_playouts", "pf_count_moves_host", "pf_serve_start", "pf_submit", "pf_poll", "pf_serve_stop", "pf_now"]
 
@@ -93,6 +93,8 @@ def load_library(path=LIB_PATH):
         "pf_perft_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, u64p]),
         "pf_perft_batch_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, u64p]),
+        "pf_perft_batch_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_void_p]),
         "pf_divide": (ctypes.c_int, [ctypes.c_void_p, P(Pos48), ctypes.c_int, P(ctypes.c_uint16), u64p,
                                      P(ctypes.c_int)]),
         "pf_move_to_uci": (ctypes.c_int, [ctypes.c_uint16, ctypes.c_char_p]),
@@ -322,6 +324,14 @@ class Engine:
         _check(self._lib.pf_perft_batch_mode(self._ctx, ctypes.c_void_p(pos_ptr), n, depth, mode,
                                              ctypes.cast(ctypes.c_void_p(out_ptr), ctypes.POINTER(ctypes.c_uint64))),
                "perft_batch_ptr")
+
+    def perft_batch_noalloc_ptr(self, pos_ptr, n, depth, out_ptr):
+        """pf_perft_batch_noalloc: the stream-ordered batch on device buffers
+        (`pos_ptr`, `out_ptr` device addresses of this context's device),
+        allocating nothing -- its levels live in the library's fixed 512 MiB
+        device array; PerftError PF_ENOMEM when they do not fit."""
+        _check(self._lib.pf_perft_batch_noalloc(self._ctx, ctypes.c_void_p(pos_ptr), n, depth,
+                                                ctypes.c_void_p(out_ptr)), "perft_batch_noalloc_ptr")
 
     def divide(self, pos, depth):
         mv = (ctypes.c_uint16 * 256)()

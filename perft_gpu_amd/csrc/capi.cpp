@@ -269,6 +269,17 @@ pf_status pf_perft_batch(pf_ctx* ctx, const pf_pos48* pos, int n, int depth, uin
     return pf_perft_batch_mode(ctx, pos, n, depth, PF_MODE_AUTO, out);
 }
 
+pf_status pf_perft_batch_noalloc(pf_ctx* ctx, const pf_pos48* pos, int n, int depth, uint64_t* out) {
+    if (!ctx || (n > 0 && (!pos || !out)) || n < 0 || check_depth(depth)) return PF_EINVAL;
+    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    if (ctx->eng.serving) return PF_EBUSY;
+    if (n == 0) return PF_OK;
+    if (hipSetDevice(ctx->eng.device) != hipSuccess) return PF_EDEVICE;
+    // both buffers in device memory of this context's device: nothing is staged
+    if (buffer_kind(pos, ctx->eng.device) != 1 || buffer_kind(out, ctx->eng.device) != 1) return PF_EINVAL;
+    return ctx->eng.perft_noalloc(pos, n, depth, out);
+}
+
 pf_status pf_divide(pf_ctx* ctx, const pf_pos48* pos, int depth, uint16_t* moves, uint64_t* counts, int* n) {
     if (!ctx || !pos || !moves || !counts || !n || depth < 1 || check_depth(depth)) return PF_EINVAL;
     const int black = pos_black_to_move(*pos);

@@ -92,6 +92,9 @@ struct Engine {
     // the same batch as one launch per level on the stream, sized on the device
     pf_status perft_stream(const pf_pos48* pos, int n, int depth, u64* out, bool dev_in = false,
                            bool dev_out = false);
+    // the stream-ordered batch on device buffers with no allocation: its
+    // control block, roots and levels in a fixed device array (noalloc.hip)
+    pf_status perft_noalloc(const pf_pos48* dpos, int n, int depth, u64* dout);
     pf_status init_tt(u64 bytes, u64 seed);
     void apply_tuning();
     pf_status load_tuning(const char* path);
