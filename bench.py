@@ -20,25 +20,31 @@ Also reported on the same box, outside the timed region:
                 launches) against MI355X's published vector rate,
                 157.3 TFLOP/s FP32 = 78.6 T lane-op/s
   plain         the same 256 positions through the plain per-position path
-  backends      batched nodes/sec per slider backend, and the level mode
-  serving       C4: Poisson arrivals (50 req/s) replayed in real time for
-                --serve-seconds, depths {3:.30,4:.30,5:.25,6:.12,7:.03},
-                32 GiB shared TT: p50/p99 latency and nodes/sec
+  backends      batched nodes/sec per slider backend, and the level modes
+  serving       C4: the whole configured stream (Poisson arrivals, 50 req/s
+                for 120 s, about 6,000 requests, depths {3:.30,4:.30,5:.25,
+                6:.12,7:.03}, 32 GiB shared TT) replayed in real time: p50/p99
+                latency overall and by depth, nodes/sec, table hits; the same
+                requests all at once (serving capacity) and one after another
+                through plain pf_perft; every count against the oracle's
   c3            one 20-30-ply middlegame position, perft 8, 64 GiB TT with
-                in-level duplicate elimination (and perft 7 against the
-                fixture, and perft 8 with the TT off); roofline over two
+                in-level duplicate elimination, against the oracle's perft 8
+                (and perft 7, and perft 8 with the TT off); roofline over two
                 resources: table traffic at the live random-access and copy
                 rates, and the counting work at the vector peak; frac is
                 the larger time over the run's (tt_roofline)
   c5            the 4096 high-mobility positions (--c5-positions), perft 6,
-                128 GiB TT
+                128 GiB TT, every count against the oracle's
+  tt_tuning     the table path's launch parameters, tuned on 512 C5 roots
   c1            the initial position, perft 5, through the plain path on the
                 GPU (median of 20 runs; gpu_stream: the same as a one-root
-                stream-ordered batch) and through the oracle's Kogge-Stone
-                perft on one host thread (median of 50 runs)
-  cpu_baseline  the oracle (CPU restatement of the reference perft, fancy
-                magics) on the host cores, same C2 positions, same depth;
-                single_thread: the same on one thread
+                stream-ordered batch); on the CPU below
+  cpu_*         the oracle (CPU restatement of the reference perft) on the
+                host's CPUs (at most 16 threads, the share one GPU's job gets;
+                nproc and the model recorded): C2 in full (fancy magics; and
+                one thread), C1 (Kogge-Stone, 50 runs, one thread and split by
+                root move), the first two C3 root moves at perft 7, C4's first
+                seconds of requests offline, and 64 C5 roots at perft 6
 """
 import argparse
 import json
@@ -57,15 +63,64 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(positions, depth, gpu_counts, budget_s=20.0):
+def load_goldens():
+    """Oracle goldens: tests/golden/workloads.json (C2 perft 1-5, C3 perft
+    1-7, ...) and, when present, tests/golden/deep.json (C3 perft 8 divided by
+    root move, perft 6 of all 4096 C5 positions, every C4 request), the
+    latter under the keys c3_deep, c5_deep and c4_deep."""
+    gdir = os.path.join(ROOT, "tests", "golden")
+    with open(os.path.join(gdir, "workloads.json")) as f:
+        g = json.load(f)
+    deep = os.path.join(gdir, "deep.json")
+    if os.path.exists(deep):
+        with open(deep) as f:
+            d = json.load(f)
+        if "perft8" in d.get("c3", {}):
+            g["c3_deep"] = d["c3"]
+        if "total" in d.get("c5", {}):
+            g["c5_deep"] = d["c5"]
+        if "requests" in d.get("c4", {}):
+            g["c4_deep"] = d["c4"]
+    return g
+
+
+def host_cpus():
+    """CPUs this process may run on, the threads the CPU legs use (the
+    machine grants one GPU's job a share of at most 16 CPUs), and the model."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "allowed": allowed, "threads": max(1, min(allowed, 16)), "model": model}
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    return Oracle()
+
+
+def _rows(positions):
+    import numpy as np
+    from perft_gpu_amd import POS_DTYPE
+    return np.array([p.as_tuple() for p in positions], dtype=POS_DTYPE)
+
+
+def cpu_baseline(positions, depth, gpu_counts, threads, budget_s=20.0):
     """Oracle (plain C restatement of the reference CPU perft, fancy magics)
     on the host cores: test infrastructure used here only as the CPU
     baseline, never as the measured path."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_lib import Oracle
-    orc = Oracle()
+    orc = _oracle()
     orc.set_backend(2)
-    threads = max(1, min(os.cpu_count() or 1, 16))
     t = time.perf_counter()
     probe = orc.perft_batch(positions[:4], depth - 1, nthreads=1)
     dt = max(time.perf_counter() - t, 1e-6)
@@ -93,61 +148,153 @@ def cpu_baseline(positions, depth, gpu_counts, budget_s=20.0):
                               "sample": "first 4 C2 positions, perft(%d) each, one pass" % (depth - 1)}}
 
 
-def cpu_c1(fen, depth, want, reps=50):
+def _pcts(times):
+    v = sorted(times)
+    return {"median_s": v[len(v) // 2], "p10_s": v[int(0.1 * (len(v) - 1))], "p90_s": v[int(0.9 * (len(v) - 1))]}
+
+
+def cpu_c1(fen, depth, want, threads, reps=50):
     """C1 on the CPU as the configuration names it: the oracle's Kogge-Stone
-    perft of one position on one thread, `reps` runs, median time."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_lib import Oracle
-    orc = Oracle()
-    old = orc.set_backend(0)  # Kogge-Stone, for the calling thread
-    times, ok = [], True
+    perft of one position, `reps` runs each on one thread and on `threads`
+    threads (split by root move), median / p10 / p90."""
+    orc = _oracle()
+    old = orc.set_backend(0)  # Kogge-Stone, for the calling thread (workers inherit it)
     try:
+        root = orc.pos(fen)
+        kids = _rows(orc.children(root))
+        one, many, ok = [], [], True
         for _ in range(reps):
             t = time.perf_counter()
-            got = orc.perft_fen(fen, depth)
-            times.append(time.perf_counter() - t)
+            got = orc.perft(root, depth)
+            one.append(time.perf_counter() - t)
+            ok = ok and got == want
+            t = time.perf_counter()
+            got = int(orc.perft_batch(kids, depth - 1, nthreads=threads).sum())
+            many.append(time.perf_counter() - t)
             ok = ok and got == want
     finally:
         orc.set_backend(old)
-    med = sorted(times)[len(times) // 2]
-    return {"value": want / med, "unit": "nodes/s", "median_s": med, "runs": reps, "cores": 1, "kind": "port",
-            "backend": "ks", "counts_match": ok}
+    p1, pm = _pcts(one), _pcts(many)
+    return {"value": want / p1["median_s"], "unit": "nodes/s", "runs": reps, "cores": 1, "kind": "port",
+            "backend": "ks", "counts_match": ok, **p1,
+            "all_cores": {"value": want / pm["median_s"], "unit": "nodes/s", "cores": threads, "runs": reps,
+                          "split": "by root move (%d subtrees of perft %d)" % (len(kids), depth - 1), **pm}}
 
 
-def run_serving(pg, W, seconds, backend, paced=True):
+def cpu_c3_subset(c3_pos, golden, threads, subtrees=2):
+    """C3 on the CPU: the first `subtrees` root moves' perft 7 (of the 36),
+    each split into its perft-6 children over the threads; against the
+    golden divide of perft 8."""
+    orc = _oracle()
+    orc.set_backend(2)
+    root = orc.from48(c3_pos)
+    moves = orc.moves(root)[:subtrees]
+    tasks = []
+    for m in moves:
+        tasks.extend(orc.children(orc.make(root, m)))
+    t = time.perf_counter()
+    got = orc.perft_batch(_rows(tasks), 6, nthreads=threads)
+    el = time.perf_counter() - t
+    nodes = int(got.sum())
+    ok = None
+    if golden:
+        ok = nodes == sum(golden["divide_d8"]["%d" % m] for m in moves)
+    return {"value": nodes / el, "unit": "nodes/s", "cores": threads, "kind": "port", "seconds": el,
+            "nodes": nodes, "counts_match_golden": ok,
+            "sample": "perft 7 of the first %d of the C3 position's %d root moves, split into %d perft-6 subtrees, "
+                      "oracle fancy magics" % (len(moves), len(orc.moves(root)), len(tasks))}
+
+
+def cpu_c4_offline(W, golden, threads, seconds=4.0):
+    """C4 offline on the CPU: the requests that arrive in the stream's first
+    `seconds`, no pacing, every request split by root move; nodes/s only."""
+    orc = _oracle()
+    orc.set_backend(2)
+    reqs = [r for r in W.serving_requests(W.C4) if r[0] < seconds]
+    by_depth, owner = {}, {}
+    for i, (_, d, sd) in enumerate(reqs):
+        p = orc.from48(W.request_position(sd))
+        if d <= 1:
+            by_depth.setdefault(0, []).append((i, p))
+            continue
+        for c in orc.children(p):
+            by_depth.setdefault(d - 1, []).append((i, c))
+    per = [0] * len(reqs)
+    t = time.perf_counter()
+    for d, items in sorted(by_depth.items(), reverse=True):
+        got = orc.perft_batch(_rows([c for _, c in items]), d, nthreads=threads)
+        for (i, _), n in zip(items, got):
+            per[i] += int(n)
+    el = time.perf_counter() - t
+    ok = None
+    if golden:
+        ok = per == [r[3] for r in golden["requests"][:len(reqs)]]
+    return {"value": sum(per) / el, "unit": "nodes/s", "cores": threads, "kind": "port", "seconds": el,
+            "nodes": sum(per), "requests": len(reqs), "counts_match_golden": ok,
+            "sample": "the %d C4 requests arriving in the first %.0f s, processed offline, split by root move, "
+                      "oracle fancy magics" % (len(reqs), seconds)}
+
+
+def cpu_c5_subset(c5_positions, golden, threads, n=64):
+    """C5 on the CPU: perft 6 of the first n of the 4096 positions."""
+    orc = _oracle()
+    orc.set_backend(2)
+    t = time.perf_counter()
+    got = orc.perft_batch(c5_positions[:n], 6, nthreads=threads)
+    el = time.perf_counter() - t
+    ok = None
+    if golden:
+        ok = [int(x) for x in got] == golden["perft6"][:n]
+    return {"value": float(got.sum()) / el, "unit": "nodes/s", "cores": threads, "kind": "port", "seconds": el,
+            "nodes": int(got.sum()), "counts_match_golden": ok,
+            "sample": "perft 6 of the first %d C5 positions, oracle fancy magics" % n}
+
+
+def run_serving(pg, W, seconds, backend, paced=True, setup=None):
     """Replay the C4 stream's first `seconds` of arrivals; paced=False submits
-    everything at once (capacity)."""
+    everything at once (capacity); setup(engine) runs before serving starts."""
     cfg = W.C4
     reqs = W.serving_requests(cfg)
     reqs = [r for r in reqs if r[0] < seconds]
     poss = [W.request_position(s) for _, _, s in reqs]
     eng = pg.Engine(backend=backend, arena_bytes=96 << 30, tt_bytes=cfg["tt_bytes"], zobrist_seed=cfg["seed"])
-    eng.serve_start(max_jobs=256)
     results = {}
     done = threading.Event()
+    try:
+        if setup:
+            setup(eng)
+        eng.serve_start(max_jobs=256)
 
-    def poller():
-        while not done.is_set() or len(results) < len(reqs):
-            for r in eng.poll():
-                results[r["tag"]] = r
-            if len(results) >= len(reqs):
-                break
-            time.sleep(0.0002)
+        def poller():
+            while not done.is_set() or len(results) < len(reqs):
+                for r in eng.poll():
+                    results[r["tag"]] = r
+                if len(results) >= len(reqs):
+                    break
+                time.sleep(0.0002)
 
-    th = threading.Thread(target=poller)
-    th.start()
-    t0 = time.perf_counter()
-    c0 = eng.now()
-    for i, ((t, d, _), p) in enumerate(zip(reqs, poss)):
-        wait = t - (time.perf_counter() - t0)
-        if paced and wait > 0:
-            time.sleep(wait)
-        eng.submit(p, d, i)
-    done.set()
-    th.join(timeout=600)
-    st = eng.stats()
-    eng.serve_stop()
-    eng.close()
+        th = threading.Thread(target=poller, daemon=True)
+        th.start()
+        try:
+            t0 = time.perf_counter()
+            c0 = eng.now()
+            next_log = 10.0
+            for i, ((t, d, _), p) in enumerate(zip(reqs, poss)):
+                wait = t - (time.perf_counter() - t0)
+                if paced and wait > 0:
+                    time.sleep(wait)
+                eng.submit(p, d, i)
+                if paced and t >= next_log:
+                    log("serving: %.0f s of arrivals, %d submitted, %d done" % (t, i + 1, len(results)))
+                    next_log += 10.0
+        finally:
+            done.set()
+            th.join(timeout=600)
+        st = eng.stats()
+        eng.serve_stop()
+    finally:
+        done.set()
+        eng.close()
     lat = sorted(r["done_s"] - r["submit_s"] for r in results.values())
     nodes = sum(r["nodes"] for r in results.values())
     span = max(r["done_s"] for r in results.values()) - c0
@@ -164,7 +311,8 @@ def run_serving(pg, W, seconds, backend, paced=True):
             "nodes": nodes, "nodes_per_s": nodes / span if span > 0 else None, "span_s": span,
             "arrival_rate_per_s": cfg["rate"], "seconds": seconds, "steps": st["levels_run"],
             "step_busy_s": st["gpu_ms"] * 1e-3,
-            "dup_nodes": st["dup_nodes"],
+            "dup_nodes": st["dup_nodes"], "tt_probes": st["tt_probes"], "tt_hits": st["tt_hits"],
+            "requests_by_depth": {str(d): len(v) for d, v in sorted(by_depth.items())},
             "p50_ms_by_depth": {str(d): pct(sorted(v), 0.5) for d, v in sorted(by_depth.items())},
             "p99_ms_by_depth": {str(d): pct(sorted(v), 0.99) for d, v in sorted(by_depth.items())},
             "_counts": [results[i]["nodes"] if i in results else None for i in range(len(reqs))]}
@@ -237,10 +385,13 @@ def tt_roofline(st, seconds, ceil):
             "frac": max(t_mem, t_valu) / seconds, "serial_frac": (t_mem + t_valu) / seconds}
 
 
-def run_tt_workload(pg, W, cfg, positions, depth, backend, arena, check=None, ceil=None, tt=True):
+def run_tt_workload(pg, W, cfg, positions, depth, backend, arena, check=None, ceil=None, tt=True, tune_file=None,
+                    setup=None):
     eng = pg.Engine(backend=backend, arena_bytes=arena, tt_bytes=cfg["tt_bytes"] if tt else 0,
-                    zobrist_seed=cfg["seed"])
+                    zobrist_seed=cfg["seed"], tune_file=tune_file)
     try:
+        if setup:
+            setup(eng)
         t = time.perf_counter()
         got = eng.perft_batch(positions, depth)
         el = time.perf_counter() - t
@@ -249,7 +400,8 @@ def run_tt_workload(pg, W, cfg, positions, depth, backend, arena, check=None, ce
                "nodes_per_s": float(got.sum()) / el, "tt_gib": (cfg["tt_bytes"] >> 30) if tt else 0,
                "tt_probes": st["tt_probes"], "tt_hits": st["tt_hits"], "tt_stores": st["tt_stores"],
                "dup_nodes": st["dup_nodes"], "frontier_nodes": st["frontier_nodes"], "made_nodes": st["made_nodes"],
-               "chunks": st["chunks"], "peak_arena_gib": st["peak_arena_bytes"] / 2 ** 30}
+               "chunks": st["chunks"], "graph_passes": st["graph_passes"],
+               "peak_arena_gib": st["peak_arena_bytes"] / 2 ** 30}
         if tt and ceil:
             out["roofline"] = tt_roofline(st, el, ceil)
         if check is not None:
@@ -259,13 +411,40 @@ def run_tt_workload(pg, W, cfg, positions, depth, backend, arena, check=None, ce
         eng.close()
 
 
+def table_semantics(pg, W, backend, c3, want8, tune_file, serve_seconds=20.0):
+    """The table's semantics measured (the reference's switches.h:43-48 and
+    launcher.h:62-126): C3 perft 8 with exact en-passant flagging and with
+    the file set after every double push -- in-level duplicates found,
+    nodes written, time; and serving capacity on the C4 stream's first
+    `serve_seconds` with one depth-keyed table, with four per-depth regions,
+    and with always-set flagging -- table probes, hits and hit rate."""
+    out = {"c3_d8": {}, "serving": {}}
+    for name, exact in (("exact_ep", True), ("ep_every_double_push", False)):
+        r = run_tt_workload(pg, W, W.C3, c3, 8, backend, 128 << 30, tune_file=tune_file,
+                            check=(lambda got: int(got[0]) == want8) if want8 else None,
+                            setup=lambda e, x=exact: e.set_exact_ep(x))
+        out["c3_d8"][name] = {k: r[k] for k in ("seconds", "dup_nodes", "frontier_nodes", "made_nodes",
+                                                "tt_probes", "tt_hits", "counts_match") if k in r}
+    for name, exact, regions in (("one_table", True, 0), ("per_depth_regions_4", True, 2),
+                                 ("ep_every_double_push", False, 0)):
+        r = run_serving(pg, W, serve_seconds, backend, paced=False,
+                        setup=lambda e, x=exact, k=regions: (e.set_exact_ep(x), e.set_tt_regions(k)))
+        r.pop("_counts")
+        out["serving"][name] = {k: r[k] for k in ("requests", "nodes_per_s", "tt_probes", "tt_hits", "dup_nodes",
+                                                  "span_s")}
+        out["serving"][name]["hit_rate"] = r["tt_hits"] / max(1, r["tt_probes"])
+    out["serving_sample"] = "the C4 requests arriving in the first %.0f s, all submitted at once" % serve_seconds
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--backend", default="ks")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=120.0,
+                    help="seconds of the C4 arrival stream to serve (120 = the whole configured stream)")
     ap.add_argument("--c5-positions", type=int, default=4096)
     ap.add_argument("--no-extras", action="store_true", help="skip every comparison run")
     ap.add_argument("--sections", default="batch,serving,tt",
@@ -287,18 +466,21 @@ def main():
     cfg = W.C2
     depth = cfg["depth"]
     pos = W.positions(cfg)
+    gold = load_goldens()
     eng = pg.Engine(backend=args.backend, arena_bytes=16 << 30)
     log("bench: %d positions depth %d backend %s" % (len(pos), depth, args.backend))
     tuning = None
+    import shutil
+    import tempfile
+    tune_dir = tempfile.mkdtemp()
+    tf = os.path.join(tune_dir, "tune.txt")
     if not args.no_tune:
-        # the autotuner times every launch shape on this workload and keeps the fastest
-        import tempfile
-        with tempfile.TemporaryDirectory() as td:
-            tf = os.path.join(td, "tune.txt")
-            t = time.perf_counter()
-            eng.tune(pos, depth, tf)
-            tuning = {"seconds": time.perf_counter() - t,
-                      "choice": [l.strip() for l in open(tf) if l.startswith("backend " + args.backend + " ")]}
+        # the autotuner times every launch shape of the table-free passes on
+        # this workload and keeps the fastest
+        t = time.perf_counter()
+        eng.tune(pos, depth, tf, what=pg.TUNE_SINGLE_PASS)
+        tuning = {"seconds": time.perf_counter() - t,
+                  "choice": [l.strip() for l in open(tf) if l.startswith("backend " + args.backend + " ")]}
         log("tuning: %s" % tuning)
     # the single pass PF_MODE_AUTO runs for this backend (stream until tuned)
     single = "stream"
@@ -347,13 +529,21 @@ def main():
         "config": {"workload": cfg["name"], "positions": len(pos), "depth": depth, "tt": "off",
                    "backend": args.backend, "mode": "auto: " + single, "buffers": "device",
                    "leaves_per_step": leaves,
-                   "counts_match_golden": leaves == W.C2_LEAVES, "tuning": tuning},
+                   # every position's count against the oracle's golden perft 5
+                   "counts_match_golden": counts.tolist() == gold["c2"]["perft"][str(depth)],
+                   "launch_form": ("one launch per level queued on one stream, each level sized and placed on "
+                                   "the device (the same batch in one persistent cooperative launch is "
+                                   "backends.persistent_mode_%s)" % args.backend) if single == "stream" else
+                                  "one persistent cooperative launch",
+                   "tuning": tuning},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
                      "frac": achieved / VALU_PEAK_TOPS,
                      # HBM bytes per step: FETCH_SIZE x2.00 + WRITE_SIZE x1.00 over the step's kernels
                      # (rocprofv3 --pmc, calibrated on a copy of known size;
                      # profiles/k_stream_level_ks_c2_d5_counters.md)
                      "traffic": 9.522e8,
+                     "traffic_source": "profiles/k_stream_level_ks_c2_d5_counters.md (rocprofv3 --pmc "
+                                       "FETCH_SIZE, WRITE_SIZE over one step, calibrated on a copy)",
                      "kernel": ("k_stream_roots + %d x k_stream_level<%s>" % (depth - 1, args.backend)
                                 if single == "stream" else "k_persistent<%s>" % args.backend),
                      "kernel_ms": kernel_ms,
@@ -436,6 +626,10 @@ def main():
             if "plain" in modes:
                 runs["plain_same_requests"] = run_plain_requests(pg, W, args.serve_seconds, args.backend)
             req_counts = {k: r.pop("_counts") for k, r in runs.items()}
+            if "c4_deep" in gold:  # the oracle's count of every request of the stream
+                want = [r[3] for r in gold["c4_deep"]["requests"]]
+                for k, c in req_counts.items():
+                    runs[k]["counts_match_golden"] = c == want[:len(c)]
             if "plain_same_requests" in runs:
                 for k in ("serving", "serving_capacity"):
                     if k in runs:
@@ -449,34 +643,81 @@ def main():
             line["serving_error"] = repr(e)
     if "tt" in sections:
         try:
-            import json as _j
-            with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
-                g = _j.load(f)
+            g = gold
             ceil = measure_ceilings(pg)
             log("ceilings: %s" % ceil)
+            tt_tune = None
+            if not args.no_tune:
+                # the table path's workgroups per CU and chunk share, tuned on
+                # 512 of the C5 positions at perft 6 with C5's table and arena
+                c5s = W.positions(W.C5)[:512]
+                teng = pg.Engine(backend=args.backend, arena_bytes=96 << 30, tt_bytes=W.C5["tt_bytes"],
+                                 zobrist_seed=W.C5["seed"], tune_file=tf if os.path.exists(tf) else None)
+                try:
+                    t = time.perf_counter()
+                    teng.tune(c5s, 6, tf, what=pg.TUNE_TABLE)
+                    tt_tune = {"seconds": time.perf_counter() - t, "sample": "first 512 C5 positions, perft 6",
+                               "choice": [l.strip() for l in open(tf)
+                                          if l.startswith("backend " + args.backend + " ")]}
+                finally:
+                    teng.close()
+                log("table tuning: %s" % tt_tune)
+                line["tt_tuning"] = tt_tune
+            ttf = tf if os.path.exists(tf) else None
             c3 = W.positions(W.C3)
-            line["c3"] = run_tt_workload(pg, W, W.C3, c3, 8, args.backend, 128 << 30, ceil=ceil)
+            want8 = g["c3_deep"]["perft8"] if "c3_deep" in g else None
+            line["c3"] = run_tt_workload(pg, W, W.C3, c3, 8, args.backend, 128 << 30, ceil=ceil,
+                                         check=(lambda got: int(got[0]) == want8) if want8 else None,
+                                         tune_file=ttf)
             line["c3"]["d7"] = run_tt_workload(pg, W, W.C3, c3, 7, args.backend, 128 << 30, ceil=ceil,
-                                               check=lambda got: int(got[0]) == g["c3"]["perft"]["7"][0])
+                                               check=lambda got: int(got[0]) == g["c3"]["perft"]["7"][0],
+                                               tune_file=ttf)
             line["c3"]["d8_tt_off"] = run_tt_workload(pg, W, W.C3, c3, 8, args.backend, 128 << 30, tt=False)
             line["c3"]["speedup_over_tt_off"] = line["c3"]["d8_tt_off"]["seconds"] / line["c3"]["seconds"]
             line["c3"]["counts_match_tt_off"] = line["c3"]["nodes"] == line["c3"]["d8_tt_off"]["nodes"]
+            # the same run with the host-driven levels (no captured passes)
+            line["c3"]["d8_host_levels"] = run_tt_workload(
+                pg, W, W.C3, c3, 8, args.backend, 128 << 30, tune_file=ttf,
+                check=(lambda got: int(got[0]) == want8) if want8 else None,
+                setup=lambda e: e.set_level_graphs(False))
             log("c3: %s" % line["c3"])
             if args.c5_positions > 0:
                 c5 = W.positions(W.C5)[:args.c5_positions]
+                want6 = g["c5_deep"]["perft6"] if "c5_deep" in g else g["c5"]["perft_first8_d6"]
                 line["c5"] = run_tt_workload(
                     pg, W, W.C5, c5, 6, args.backend, 96 << 30, ceil=ceil,
-                    check=lambda got: [int(x) for x in got[:8]] == g["c5"]["perft_first8_d6"][:len(got)])
+                    check=lambda got: [int(x) for x in got[:len(want6)]] == want6[:len(got)], tune_file=ttf)
+                line["c5"]["checked_positions"] = min(len(c5), len(want6))
+                if len(c5) == 4096:
+                    line["c5"]["host_levels"] = run_tt_workload(
+                        pg, W, W.C5, c5, 6, args.backend, 96 << 30, tune_file=ttf,
+                        check=lambda got: [int(x) for x in got[:len(want6)]] == want6[:len(got)],
+                        setup=lambda e: e.set_level_graphs(False))
+                    log("c5 host levels: %.3f s" % line["c5"]["host_levels"]["seconds"])
                 log("c5: %s" % line["c5"])
+            line["f2"] = table_semantics(pg, W, args.backend, c3, want8, ttf)
+            log("f2: %s" % line["f2"])
         except Exception as e:
             line["tt_runs_error"] = repr(e)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(pos, depth, counts)
+        hc = host_cpus()
+        th = hc["threads"]
+        line["cpu_baseline"] = cpu_baseline(pos, depth, counts, th)
+        line["cpu_baseline"]["host"] = hc
         log("cpu: %s" % line["cpu_baseline"])
-        if "c1" in line:
-            line["c1"]["cpu"] = cpu_c1(W.C1["fen"], W.C1["depth"], 4865609)
-            line["c1"]["gpu_over_cpu"] = line["c1"]["gpu"]["value"] / line["c1"]["cpu"]["value"]
-            log("c1 cpu: %s" % line["c1"]["cpu"])
+        c1cpu = cpu_c1(W.C1["fen"], W.C1["depth"], 4865609, th)
+        line.setdefault("c1", {})["cpu"] = c1cpu
+        if "gpu" in line["c1"]:
+            line["c1"]["gpu_over_cpu"] = line["c1"]["gpu"]["value"] / c1cpu["value"]
+            line["c1"]["gpu_over_cpu_all_cores"] = line["c1"]["gpu"]["value"] / c1cpu["all_cores"]["value"]
+        log("c1 cpu: %s" % c1cpu)
+        line["cpu_c3"] = cpu_c3_subset(W.positions(W.C3)[0], gold.get("c3_deep"), th)
+        log("c3 cpu: %s" % line["cpu_c3"])
+        line["cpu_c4"] = cpu_c4_offline(W, gold.get("c4_deep"), th)
+        log("c4 cpu: %s" % line["cpu_c4"])
+        line["cpu_c5"] = cpu_c5_subset(W.positions(W.C5), gold.get("c5_deep"), th)
+        log("c5 cpu: %s" % line["cpu_c5"])
+    shutil.rmtree(tune_dir, ignore_errors=True)
     print(json.dumps(line), flush=True)
 
 

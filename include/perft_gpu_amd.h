@@ -91,7 +91,7 @@ typedef struct pf_stats {
     uint64_t levels_run;
     uint64_t chunks;          /* depth-first chunk splits taken */
     uint64_t peak_arena_bytes;
-    uint64_t tt_probes, tt_hits, tt_stores;
+    uint64_t tt_probes, tt_hits, tt_stores;  /* while serving: probes and hits since pf_serve_start */
     double gpu_ms;            /* device time of expand kernels (HIP events) */
     double last_expand_ms;    /* average duration of the last run's expand launches */
     uint64_t last_expand_launches;
@@ -99,6 +99,7 @@ typedef struct pf_stats {
     uint64_t frontier_nodes;  /* nodes written to frontier levels (TT runs) */
     uint64_t made_nodes;      /* children made and counted by the level kernels (TT runs): hits,
                                  duplicates, written nodes and the leaf pass's children */
+    uint64_t graph_passes;    /* TT runs: chunks that ran as one captured-graph pass */
 } pf_stats;
 
 /* Replaces initGPU + MoveGeneratorBitboard::init + setupHashTables128b
@@ -158,14 +159,50 @@ pf_status pf_set_backend(pf_ctx *ctx, int backend);
  * default): the compile-time FIND_DUPLICATES_IN_BFS of switches.h:50-52 as a
  * run-time switch.  Counts are the same either way. */
 pf_status pf_set_dedup(pf_ctx *ctx, int on);
+/* Runs with a transposition table: a chunk of a level whose whole subtree is
+ * predicted to fit the free frontier memory runs as one replay of a captured
+ * hipGraph (on by default), every level below it sized and placed on the
+ * device, with no host round trip per level (the reference syncs the device
+ * after every level launch, perft_bb.h:1071-1135); off = the host-driven
+ * level loop everywhere.  Counts are the same either way. */
+pf_status pf_set_level_graphs(pf_ctx *ctx, int on);
+/* En-passant flagging of the children made on runs with a table:
+ * exact (on, the default) sets a child's en-passant file only when an enemy
+ * pawn can capture (EXACT_EN_PASSENT_FLAGGING, switches.h:43-48,
+ * MoveGeneratorBitboard.h:3528-3547); off sets it after every double push, as
+ * the reference's CPU generator does (MoveGeneratorBitboard.h:1575-1582).
+ * Counts are the same either way; keys, and so table hits and in-level
+ * duplicates, are not.  PF_EBUSY while serving. */
+pf_status pf_set_exact_ep(pf_ctx *ctx, int on);
+/* Table layout: 0 (the default) = one table for every remaining depth (the
+ * depth is verified in each entry); k = 1..4 = 2^k equal regions, an entry
+ * going to the region of its remaining depth modulo 2^k (the reference's
+ * one table per depth, launcher.h:62-126).  Empties the table.  PF_EINVAL
+ * without a table or when a region would hold fewer than 1024 buckets;
+ * pf_tt_save / pf_tt_load need layout 0; PF_EBUSY while serving. */
+pf_status pf_set_tt_regions(pf_ctx *ctx, int log2_regions);
 /* Autotuner (replaces the reference's compile-time launch shape, perft_bb.h:327-329,
  * switches.h:14-21, and its launch-depth estimate, launcher.h:1535-1563): times
- * the current backend on `sample` (n positions, `depth`) over the persistent
- * kernel's tile shapes and workgroups per CU and, with a table, the TT expand
- * kernel's workgroups per CU; keeps the fastest and, if `path` is not NULL,
- * writes every backend's choice there (the file pf_config.tune_file loads).
- * Counts are checked identical across all tried shapes (PF_EDEVICE if not). */
+ * the current backend on `sample` (n positions, `depth`) over
+ *   - the table-free single passes: the cooperative kernel's tile shapes and
+ *     workgroups per CU, the stream-ordered level kernels' forms (tile shape x
+ *     waves per SIMD, leaf and write level separately) and which of the two
+ *     passes PF_MODE_AUTO runs;
+ *   - with a table: the level driver's write- and leaf-level workgroups per CU
+ *     and, if the sample makes it chunk, the share of free memory a chunk's
+ *     next level may take;
+ * keeps the fastest and, if `path` is not NULL, writes every backend's choice
+ * there (the file pf_config.tune_file loads).
+ * Counts are checked identical across all tried shapes (PF_EDEVICE if not).
+ * With a table configured, pf_tune empties it before every timed run and
+ * leaves it empty: a table restored by pf_tt_load or filled by earlier runs is
+ * discarded, so tune before loading one. */
 pf_status pf_tune(pf_ctx *ctx, const pf_pos48 *sample, int n, int depth, const char *path);
+/* The same, restricted to one part: PF_TUNE_SINGLE_PASS, PF_TUNE_TABLE, or
+ * PF_TUNE_ALL (pf_tune).  Tuning the table path alone lets a large table
+ * sample be timed without the table-free passes running on it too. */
+typedef enum pf_tune_what { PF_TUNE_ALL = 0, PF_TUNE_SINGLE_PASS = 1, PF_TUNE_TABLE = 2 } pf_tune_what;
+pf_status pf_tune_mode(pf_ctx *ctx, const pf_pos48 *sample, int n, int depth, int what, const char *path);
 
 /* Records mode (replaces processPerftRecords, launcher.h:1578-1695): perft(depth)
  * of every record "<FEN ...> <occurrences>" of the text file `in_path`, appended

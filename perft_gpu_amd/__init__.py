@@ -22,6 +22,7 @@ CLI_PATH = os.path.join(HERE, "_build", "perft_gpu")
 PF_OK, PF_EINVAL, PF_ENOMEM, PF_EDEVICE, PF_EBUSY, PF_EAGAIN, PF_EUNAVAIL, PF_EIO = 0, -1, -2, -3, -4, -5, -6, -7
 BACKENDS = {"ks": 0, "plain": 1, "fancy": 2, "byte": 3}
 MODE_AUTO, MODE_LEVELS, MODE_PERSISTENT, MODE_STREAM = 0, 1, 2, 3
+TUNE_ALL, TUNE_SINGLE_PASS, TUNE_TABLE = 0, 1, 2
 START_FEN = "rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1"
 
 # structured dtype with the exact pf_pos48 layout, for zero-copy batches
@@ -58,7 +59,8 @@ class Stats(ctypes.Structure):
                 ("peak_arena_bytes", ctypes.c_uint64), ("tt_probes", ctypes.c_uint64), ("tt_hits", ctypes.c_uint64),
                 ("tt_stores", ctypes.c_uint64), ("gpu_ms", ctypes.c_double), ("last_expand_ms", ctypes.c_double),
                 ("last_expand_launches", ctypes.c_uint64), ("dup_nodes", ctypes.c_uint64),
-                ("frontier_nodes", ctypes.c_uint64), ("made_nodes", ctypes.c_uint64)]
+                ("frontier_nodes", ctypes.c_uint64), ("made_nodes", ctypes.c_uint64),
+                ("graph_passes", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -69,7 +71,7 @@ _lib = None
 # every symbol the C ABI declares (checked by the CPU test suite)
 EXPORTS = ["pf_init", "pf_destroy", "pf_strerror", "pf_fen_to_pos", "pf_pos_to_fen", "pf_perft",
            "pf_perft_batch", "pf_perft_batch_mode", "pf_perft_batch_noalloc", "pf_divide", "pf_move_to_uci", "pf_apply_uci",
-           "pf_set_backend", "pf_set_dedup", "pf_tune", "pf_records", "pf_tt_save", "pf_tt_load", "pf_get_stats", "pf_reset_stats",
+           "pf_set_backend", "pf_set_dedup", "pf_set_level_graphs", "pf_set_exact_ep", "pf_set_tt_regions", "pf_tune", "pf_tune_mode", "pf_records", "pf_tt_save", "pf_tt_load", "pf_get_stats", "pf_reset_stats",
            "pf_sync", "pf_probe", "pf_playouts", "pf_count_moves_host", "pf_serve_start", "pf_submit", "pf_poll", "pf_serve_stop", "pf_now"]
 
 
@@ -109,7 +111,12 @@ def load_library(path=LIB_PATH):
                                        ctypes.c_int, ctypes.c_void_p]),
         "pf_count_moves_host": (ctypes.c_int, [P(Pos48)]),
         "pf_set_dedup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "pf_set_level_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "pf_set_exact_ep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "pf_set_tt_regions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
         "pf_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
+        "pf_tune_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_char_p]),
         "pf_records": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, u64p, u64p]),
         "pf_tt_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
         "pf_tt_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
@@ -280,12 +287,13 @@ class Engine:
         _check(self._lib.pf_set_backend(self._ctx, BACKENDS[backend] if isinstance(backend, str) else int(backend)),
                "set_backend")
 
-    def tune(self, positions, depth, path=None):
+    def tune(self, positions, depth, path=None, what=TUNE_ALL):
         """Autotune the current backend's launch parameters on a sample
-        (pf_tune); optionally write the tuning file."""
+        (pf_tune_mode: TUNE_ALL, TUNE_SINGLE_PASS or TUNE_TABLE); optionally
+        write the tuning file."""
         arr = positions_array(positions)
-        _check(self._lib.pf_tune(self._ctx, arr.ctypes.data, len(arr), depth,
-                                 path.encode() if path else None), "tune")
+        _check(self._lib.pf_tune_mode(self._ctx, arr.ctypes.data, len(arr), depth, what,
+                                      path.encode() if path else None), "tune")
 
     def records(self, path, depth, batch=0):
         """Records mode (pf_records): returns (weighted_total, records_done)."""
@@ -301,6 +309,18 @@ class Engine:
     def tt_load(self, path):
         """Replace the transposition table with a saved one (pf_tt_load)."""
         _check(self._lib.pf_tt_load(self._ctx, path.encode()), "tt_load(%r)" % path)
+
+    def set_level_graphs(self, on):
+        """pf_set_level_graphs: table runs use captured-graph passes (default on)."""
+        _check(self._lib.pf_set_level_graphs(self._ctx, 1 if on else 0), "set_level_graphs")
+
+    def set_exact_ep(self, on):
+        """pf_set_exact_ep: children flag en passant only when capturable (default)."""
+        _check(self._lib.pf_set_exact_ep(self._ctx, 1 if on else 0), "set_exact_ep")
+
+    def set_tt_regions(self, log2_regions):
+        """pf_set_tt_regions: 0 one table for every depth, k: 2^k per-depth regions."""
+        _check(self._lib.pf_set_tt_regions(self._ctx, log2_regions), "set_tt_regions")
 
     def set_dedup(self, on):
         _check(self._lib.pf_set_dedup(self._ctx, 1 if on else 0), "set_dedup")

@@ -2,6 +2,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -12,6 +13,9 @@
 struct pf_ctx {
     pf::Engine eng;
     pf::Server* srv = nullptr;
+    // guards srv for pf_submit / pf_poll, which run without the context lock
+    // while pf_serve_stop / pf_destroy may delete it
+    std::mutex srv_mu;
 };
 
 using namespace pf;
@@ -90,8 +94,12 @@ void pf_destroy(pf_ctx* ctx) {
     if (!ctx) return;
     if (ctx->srv) {
         ctx->srv->stop();
-        delete ctx->srv;
-        ctx->srv = nullptr;
+        pf::Server* s = nullptr;
+        {
+            std::lock_guard<std::mutex> g(ctx->srv_mu);
+            std::swap(s, ctx->srv);
+        }
+        delete s;
     }
     ctx->eng.destroy();
     delete ctx;
@@ -333,13 +341,45 @@ pf_status pf_set_backend(pf_ctx* ctx, int backend) {
     return PF_OK;
 }
 
-pf_status pf_tune(pf_ctx* ctx, const pf_pos48* sample, int n, int depth, const char* path) {
+pf_status pf_tune_mode(pf_ctx* ctx, const pf_pos48* sample, int n, int depth, int what, const char* path) {
     if (!ctx || !sample || n <= 0) return PF_EINVAL;
+    if (what == PF_TUNE_ALL) what = PF_TUNE_SINGLE_PASS | PF_TUNE_TABLE;
+    if (what & ~(PF_TUNE_SINGLE_PASS | PF_TUNE_TABLE)) return PF_EINVAL;
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
     if (ctx->srv) return PF_EBUSY;
-    pf_status s = ctx->eng.autotune(sample, n, depth);
+    pf_status s = ctx->eng.autotune(sample, n, depth, what);
     if (s == PF_OK && path) s = ctx->eng.save_tuning(path);
     return s;
+}
+
+pf_status pf_tune(pf_ctx* ctx, const pf_pos48* sample, int n, int depth, const char* path) {
+    return pf_tune_mode(ctx, sample, n, depth, PF_TUNE_ALL, path);
+}
+
+pf_status pf_set_exact_ep(pf_ctx* ctx, int on) {
+    if (!ctx) return PF_EINVAL;
+    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    if (ctx->eng.serving) return PF_EBUSY;
+    ctx->eng.exact_ep = on != 0;
+    return PF_OK;
+}
+
+pf_status pf_set_tt_regions(pf_ctx* ctx, int log2_regions) {
+    if (!ctx || log2_regions < 0 || log2_regions > 4) return PF_EINVAL;
+    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    if (ctx->eng.serving) return PF_EBUSY;
+    if (!ctx->eng.d_tt) return PF_EINVAL;
+    if ((ctx->eng.tt_buckets >> log2_regions) < 1024) return PF_EINVAL;
+    ctx->eng.tt_split = log2_regions;
+    ctx->eng.drop_plan_graphs();  // captured passes hold the table's mask word
+    return ctx->eng.tt_clear();
+}
+
+pf_status pf_set_level_graphs(pf_ctx* ctx, int on) {
+    if (!ctx) return PF_EINVAL;
+    std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
+    ctx->eng.tt_graphs = on != 0;
+    return PF_OK;
 }
 
 pf_status pf_set_dedup(pf_ctx* ctx, int on) {
@@ -368,11 +408,13 @@ pf_status pf_get_stats(pf_ctx* ctx, pf_stats* out) {
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
     *out = ctx->eng.st;
     if (ctx->srv) {  // the serving loop keeps its own counters
-        u64 steps = 0, segs = 0, dups = 0;
+        u64 steps = 0, segs = 0, dups = 0, probes = 0, hits = 0;
         double busy = 0;
-        ctx->srv->counters(&steps, &segs, &busy, &dups);
+        ctx->srv->counters(&steps, &segs, &busy, &dups, &probes, &hits);
         out->levels_run += steps;
         out->dup_nodes += dups;
+        out->tt_probes = probes;
+        out->tt_hits = hits;
         out->gpu_ms += busy * 1e3;  // step graphs, launch to completion
     }
     return PF_OK;
@@ -383,13 +425,16 @@ pf_status pf_serve_start(pf_ctx* ctx, int max_jobs, uint64_t job_bytes) {
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
     if (ctx->srv) return PF_EBUSY;
     if (ctx->eng.arena_top != 0) return PF_EBUSY;
-    ctx->srv = new (std::nothrow) Server(ctx->eng, max_jobs, job_bytes);
-    if (!ctx->srv) return PF_ENOMEM;
-    const pf_status s = ctx->srv->start();
+    Server* srv = new (std::nothrow) Server(ctx->eng, max_jobs, job_bytes);
+    if (!srv) return PF_ENOMEM;
+    const pf_status s = srv->start();
     if (s != PF_OK) {
-        delete ctx->srv;
-        ctx->srv = nullptr;
+        delete srv;
         return s;
+    }
+    {
+        std::lock_guard<std::mutex> gs(ctx->srv_mu);
+        ctx->srv = srv;
     }
     ctx->eng.serving = true;
     return PF_OK;
@@ -397,12 +442,14 @@ pf_status pf_serve_start(pf_ctx* ctx, int max_jobs, uint64_t job_bytes) {
 
 pf_status pf_submit(pf_ctx* ctx, const pf_pos48* pos, int depth, uint64_t tag) {
     if (!ctx || !pos) return PF_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->srv_mu);
     if (!ctx->srv) return PF_EINVAL;
     return ctx->srv->submit(*pos, depth, tag);
 }
 
 int pf_poll(pf_ctx* ctx, pf_result* out, int max) {
     if (!ctx || !out || max < 0) return PF_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->srv_mu);
     if (!ctx->srv) return PF_EINVAL;
     return ctx->srv->poll(out, max);
 }
@@ -411,9 +458,15 @@ pf_status pf_serve_stop(pf_ctx* ctx) {
     if (!ctx) return PF_EINVAL;
     std::lock_guard<std::recursive_mutex> g(ctx->eng.mu);
     if (!ctx->srv) return PF_OK;
+    // drain first (pf_submit sees PF_EBUSY, pf_poll still answers), then
+    // unpublish the server under srv_mu before deleting it
     ctx->srv->stop();
-    delete ctx->srv;
-    ctx->srv = nullptr;
+    pf::Server* s = nullptr;
+    {
+        std::lock_guard<std::mutex> gs(ctx->srv_mu);
+        std::swap(s, ctx->srv);
+    }
+    delete s;
     ctx->eng.serving = false;
     return PF_OK;
 }

@@ -70,7 +70,6 @@ pf_status Engine::init(const pf_config& cfg) {
         for (int m = 0; m < 2; m++) {
             grid[b][m] = expand_grid(b, m, 0, n_cu);
             grid_small[b][m] = expand_grid(b, m, 1, n_cu);
-            sgrid[b][m] = stream_grid(b, m, n_cu);
         }
     apply_tuning();
     return ensure_sums(1024);
@@ -90,6 +89,12 @@ void Engine::destroy() {
     hipFree(d_z);
     hipFree(d_ttstats);
     d_tt = nullptr; d_z = nullptr; d_ttstats = nullptr; tt_buckets = 0;
+    drop_plan_graphs();
+    hipFree(d_plan);
+    hipFree(d_ptab);
+    hipHostFree(h_plan);
+    hipHostFree(h_pstat);
+    d_plan = nullptr; d_ptab = nullptr; h_plan = nullptr; h_pstat = nullptr;
     hipHostFree(h_pinned);
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
@@ -301,7 +306,8 @@ pf_status Engine::perft_stream(const pf_pos48* pos, int n, int d, u64* out, bool
     p.ctl = d_pctl;
     p.sums = d_sums;
     if (timing && hipEventRecord(ev0, stream) != hipSuccess) return fail(PF_EDEVICE);
-    if (launch_stream(backend, DevTables{d_tab}, p, sgrid[backend], stream) != hipSuccess) return fail(PF_EDEVICE);
+    const int var[2] = {tune[backend].s_leaf, tune[backend].s_write};
+    if (launch_stream(backend, DevTables{d_tab}, p, sgrid[backend], var, stream) != hipSuccess) return fail(PF_EDEVICE);
     if (timing && hipEventRecord(ev1, stream) != hipSuccess) return fail(PF_EDEVICE);
     st.kernel_launches += (u64)d;  // the root count and d - 1 levels
     st.levels_run += (u64)(d - 1);

@@ -12,6 +12,7 @@
 namespace pf {
 
 constexpr u64 kAlignTT = 256;
+enum TuneWhat : int { kTuneSinglePass = 1, kTuneTable = 2 };
 
 struct Engine {
     std::recursive_mutex mu;
@@ -42,8 +43,12 @@ struct Engine {
     struct Tune {
         int p_variant = 0;    // persistent kernel tile shape (kernels.h)
         int p_bpc = 0;        // persistent kernel workgroups per CU
-        int tt_bpc = 0;       // TT expand kernel workgroups per CU
+        int tt_bpc = 0;       // TT write-level kernel workgroups per CU
         int single_pass = 0;  // table-free batches under PF_MODE_AUTO: 0 stream-ordered levels, 1 cooperative
+        int s_leaf = kStreamLeafDefault;    // stream-ordered leaf-level kernel form (stream.hip)
+        int s_write = kStreamWriteDefault;  // stream-ordered write-level kernel form
+        int tt_leaf_bpc = 0;  // TT leaf-level kernel workgroups per CU
+        int tt_chunk_pct = 50;  // TT level driver: a chunk's next level may take this share of the free arena
     };
     Tune tune[4];
     PersistCtl* d_pctl = nullptr;
@@ -65,6 +70,19 @@ struct Engine {
     int tt_grid[4] = {};       // TT write levels (tuned workgroups per CU x CUs)
     int tt_leaf_grid[4] = {};  // TT leaf levels (the leaf kernel's full occupancy)
     bool dedup = true;  // in-level duplicate detection on TT runs
+    bool exact_ep = true;  // children flag en passant only when capturable (else after every double push)
+    int tt_split = 0;      // log2 of the table's per-depth regions (0: one table for every depth)
+    u64 tt_mask_word() const { return (tt_buckets - 1) | ((u64)tt_split << 56); }
+    // Device-planned table passes (engine_tt.cpp): a chunk whose subtree is
+    // predicted to fit the free arena runs as one replay of a captured graph.
+    bool tt_graphs = true;
+    TTPlanCtl* d_plan = nullptr;
+    StepTable* d_ptab = nullptr;
+    TTPlanParams* h_plan = nullptr;   // pinned
+    TTPlanStatus* h_pstat = nullptr;  // pinned
+    hipGraph_t plan_graph[4][kPlanMaxLevels] = {};
+    hipGraphExec_t plan_exec[4][kPlanMaxLevels] = {};
+    double plan_bpc[kPlanMaxLevels] = {};  // predicted pass bytes per child of the chunk, by remaining depth
 
     struct TLevel {  // one breadth-first level with TT bookkeeping (80 B/node)
         Board* b;
@@ -99,7 +117,9 @@ struct Engine {
     void apply_tuning();
     pf_status load_tuning(const char* path);
     pf_status save_tuning(const char* path) const;
-    pf_status autotune(const pf_pos48* pos, int n, int depth);
+    // what: kTuneSinglePass (the table-free single passes) | kTuneTable (the
+    // table's level driver, when one is configured)
+    pf_status autotune(const pf_pos48* pos, int n, int depth, int what);
     pf_status probe(int kind, u64 bytes, u64 iters, double* out);
 
   private:
@@ -114,6 +134,13 @@ struct Engine {
     pf_status expand_tt(int mode, const TLevel& lv, u64 off, u64 m, TLevel* nx, u64 cap, int L, u64* written,
                         u64* next_total);
     pf_status descend_tt(const TLevel& lv, u64 off, u64 m, u64 n_next, int L);
+    pf_status descend_tt_host(const TLevel& lv, u64 off, u64 m, u64 n_next, int L);
+    pf_status run_plan(const TLevel& lv, u64 off, u64 m, u64 n_next, int L);
+    pf_status plan_exec_for(int r, hipGraphExec_t* out);
+    double plan_bytes_per_child(int r, double bf) const;
+  public:
+    void drop_plan_graphs();
+  private:
   public:
     pf_status perft_levels_tt(const pf_pos48* pos, int n, int depth, u64* out);
     pf_status tt_clear();

@@ -40,8 +40,120 @@ pf_status Engine::init_tt(u64 bytes, u64 seed) {
     PF_HIP(hipMemcpy(d_z, &h_z, sizeof(ZTables), hipMemcpyHostToDevice));
     PF_HIP(hipMalloc(&d_ttstats, 8 * sizeof(unsigned long long)));
     PF_HIP(hipMemset(d_ttstats, 0, 8 * sizeof(unsigned long long)));
+    // the planned passes' control block, step table and pinned parameters
+    PF_HIP(hipMalloc(&d_plan, sizeof(TTPlanCtl)));
+    PF_HIP(hipMalloc(&d_ptab, sizeof(StepTable)));
+    PF_HIP(hipMemset(d_ptab, 0, sizeof(StepTable)));
+    PF_HIP(hipHostMalloc(&h_plan, sizeof(TTPlanParams), hipHostMallocDefault));
+    PF_HIP(hipHostMalloc(&h_pstat, sizeof(TTPlanStatus), hipHostMallocDefault));
+    memset(h_plan, 0, sizeof(TTPlanParams));
+    memset(h_pstat, 0, sizeof(TTPlanStatus));
     apply_tuning();
     return tt_clear();
+}
+
+void Engine::drop_plan_graphs() {
+    for (int b = 0; b < 4; b++)
+        for (int r = 0; r < kPlanMaxLevels; r++) {
+            if (plan_exec[b][r]) hipGraphExecDestroy(plan_exec[b][r]);
+            if (plan_graph[b][r]) hipGraphDestroy(plan_graph[b][r]);
+            plan_exec[b][r] = nullptr;
+            plan_graph[b][r] = nullptr;
+        }
+}
+
+// The captured pass for chunks of remaining depth r with the current backend
+// and launch grids (captured on first use, kept until the grids change).
+pf_status Engine::plan_exec_for(int r, hipGraphExec_t* out) {
+    if (r < 3 || r >= kPlanMaxLevels) return PF_EINVAL;
+    if (!plan_exec[backend][r]) {
+        const int wg = std::min(tt_grid[backend], step_max_bpc(backend, 0) * n_cu);
+        const int lg = std::min(tt_leaf_grid[backend], step_max_bpc(backend, 1) * n_cu);
+        if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return PF_EDEVICE;
+        const hipError_t e = capture_plan_pass(backend, d_tab, d_z, d_plan, d_ptab, h_plan, h_pstat, d_tt,
+                                               tt_mask_word(), d_ttstats, r, wg, lg, n_cu * 4, stream);
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(stream, &g);
+        if (e != hipSuccess || ee != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return PF_EDEVICE;
+        }
+        hipGraphExec_t x = nullptr;
+        if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+            hipGraphDestroy(g);
+            return PF_EDEVICE;
+        }
+        plan_graph[backend][r] = g;
+        plan_exec[backend][r] = x;
+    }
+    *out = plan_exec[backend][r];
+    return PF_OK;
+}
+
+// Predicted memory of a pass per child of its chunk: what the last passes of
+// this remaining depth used, or before any, 112 B per written node (80 B of
+// node, 32 B of duplicate table and list) over levels growing by the
+// branching factor, plus a third.
+double Engine::plan_bytes_per_child(int r, double bf) const {
+    if (plan_bpc[r] > 0) return plan_bpc[r];
+    const double b = std::min(48.0, std::max(8.0, bf));
+    double sum = 0, lvl = 1;
+    for (int k = 0; k <= r - 3; k++) {
+        sum += lvl;
+        lvl *= b;
+    }
+    return sum * 112.0 * 4.0 / 3.0 + 64.0;
+}
+
+// Parents [off, off + m) of level `lv` (level L; n_next children) as one
+// replay of the captured pass.  PF_ENOMEM: it did not fit, and had no effect.
+pf_status Engine::run_plan(const TLevel& lv, u64 off, u64 m, u64 n_next, int L) {
+    const int r = depth - L;
+    hipGraphExec_t ex = nullptr;
+    pf_status s = plan_exec_for(r, &ex);
+    if (s != PF_OK) return s;
+    TTLevelRec top = {};
+    top.b = lv.b + off;
+    top.root = lv.root + off;
+    top.k1 = lv.k1 + off;
+    top.k2 = lv.k2 + off;
+    top.acc = lv.acc + off;
+    top.n = m;
+    h_plan->top = top;
+    h_plan->arena = arena + arena_top;
+    h_plan->arena_bytes = arena_bytes - arena_top;
+    h_plan->depth = r;
+    h_plan->ply = L;
+    h_plan->dedup = dedup ? 1 : 0;
+    h_plan->ep_always = exact_ep ? 0 : 1;
+    if (timing) PF_HIP(hipEventRecord(ev0, stream));
+    PF_HIP(hipGraphLaunch(ex, stream));
+    if (timing) PF_HIP(hipEventRecord(ev1, stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    const TTPlanStatus ps = *h_pstat;
+    st.peak_arena_bytes = std::max<u64>(st.peak_arena_bytes, arena_top + ps.peak);
+    const double used = (double)ps.peak / (double)std::max<u64>(n_next, 1);
+    if (ps.err) {
+        // the pass stopped before its writes reached anything outside its own
+        // memory: predict more next time and let the caller redo this chunk
+        plan_bpc[r] = std::max(plan_bpc[r], plan_bytes_per_child(r, (double)n_next / (double)m)) * 2.0;
+        return PF_ENOMEM;
+    }
+    plan_bpc[r] = std::max(used * 1.15, plan_bpc[r] * 0.7);
+    st.graph_passes++;
+    st.kernel_launches += (u64)(5 * (r - 2) + 3);
+    st.levels_run += (u64)(r - 1);
+    st.made_nodes += ps.made;
+    st.frontier_nodes += ps.frontier;
+    st.dup_nodes += ps.dups;
+    if (timing) {
+        float ms = 0;
+        PF_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+        st.gpu_ms += ms;
+        run_expand_ms += ms;
+        run_expand_launches++;
+    }
+    return PF_OK;
 }
 
 pf_status Engine::tt_clear() {
@@ -96,7 +208,8 @@ pf_status Engine::expand_tt(int mode, const TLevel& lv, u64 off, u64 m, TLevel* 
     }
     a.out_count = d_ctrl + 2;
     a.tt = d_tt;
-    a.tt_mask = tt_buckets - 1;
+    a.tt_mask = tt_mask_word();
+    a.ep_always = exact_ep ? 0 : 1;
     a.child_depth = depth - (L + 1);
     a.mode = mode;
     a.tile_ctr = d_ctrl + 1;
@@ -143,7 +256,43 @@ static u64 dup_slots(bool on, u64 n, int ply) {
 }
 
 // Parents [off, off + m) of level `lv` (level L) have n_next children in all.
+// Pieces whose predicted subtree fits the free arena run as planned passes
+// (one graph replay each, no host round trip per level); what does not fit,
+// or overflows its pass, takes the host-driven levels below.
 pf_status Engine::descend_tt(const TLevel& lv, u64 off, u64 m, u64 n_next, int L) {
+    if (!m) return PF_OK;
+    const int r = depth - L;
+    if (!tt_graphs || !d_plan || r < 3 || r >= kPlanMaxLevels) return descend_tt_host(lv, off, m, n_next, L);
+    u64 a = off, left = m, left_next = n_next;
+    bool split = false;
+    while (left) {
+        const u64 free_b = arena_bytes - arena_top;
+        const double bpc = plan_bytes_per_child(r, (double)left_next / (double)left);
+        const double fit = (double)free_b * 0.9 / bpc;  // children whose passes fit
+        u64 len = left, nn = left_next;
+        if ((double)left_next > fit) {
+            len = std::max<u64>(1, (u64)((double)left * fit / (double)left_next));
+            if (len < left) {
+                pf_status s = range_count(lv.b + a, len, &nn);
+                if (s != PF_OK) return s;
+                if (!split) st.chunks++;  // this level runs depth-first in pieces
+                split = true;
+            }
+        }
+        pf_status s = PF_ENOMEM;
+        if ((double)nn * bpc <= (double)free_b) s = run_plan(lv, a, len, nn, L);
+        if (s == PF_ENOMEM) s = descend_tt_host(lv, a, len, nn, L);
+        if (s != PF_OK) return s;
+        a += len;
+        left -= len;
+        left_next -= nn;
+    }
+    return PF_OK;
+}
+
+// The host-driven form: write the next level (or chunk this one by the next
+// level's size), descend, upsweep.
+pf_status Engine::descend_tt_host(const TLevel& lv, u64 off, u64 m, u64 n_next, int L) {
     if (!m) return PF_OK;
     if (depth - L == 2) {
         st.made_nodes += n_next;
@@ -184,7 +333,7 @@ pf_status Engine::descend_tt(const TLevel& lv, u64 off, u64 m, u64 n_next, int L
             u.n = (u32)written;
             u.ndup = (u32)nx.ndup;
             u.depth = depth - (L + 1);
-            if (launch_upsweep(u, d_tt, tt_buckets - 1, d_ttstats + 2, stream) != hipSuccess) s = PF_EDEVICE;
+            if (launch_upsweep(u, d_tt, tt_mask_word(), d_ttstats + 2, stream) != hipSuccess) s = PF_EDEVICE;
             st.kernel_launches++;
         }
         release(mark);
@@ -192,7 +341,9 @@ pf_status Engine::descend_tt(const TLevel& lv, u64 off, u64 m, u64 n_next, int L
     }
     if (m == 1) return PF_ENOMEM;
     st.chunks++;
-    const u64 pieces = std::max<u64>(2, (need + free_b - 1) / std::max<u64>(free_b / 2, 1) + 1);
+    // each piece's next level may take the tuned share of the free arena
+    const u64 budget = std::max<u64>(free_b / 100 * (u64)tune[backend].tt_chunk_pct, 1);
+    const u64 pieces = std::max<u64>(2, (need + budget - 1) / budget + 1);
     const u64 step = (m + pieces - 1) / pieces;
     for (u64 a = off; a < off + m; a += step) {
         const u64 len = std::min(step, off + m - a);
@@ -232,7 +383,7 @@ pf_status Engine::perft_levels_tt(const pf_pos48* pos, int n, int d, u64* out) {
         hipMemcpyAsync(r.k2, h2.data(), (size_t)n * 8, hipMemcpyHostToDevice, stream) != hipSuccess ||
         hipMemsetAsync(d_total, 0, 8, stream) != hipSuccess)
         return fail(PF_EDEVICE);
-    if (launch_roots_tt(backend, d_tab, r.b, r.k1, r.k2, (u32)n, d, d_tt, tt_buckets - 1, r.acc, d_total, d_ttstats,
+    if (launch_roots_tt(backend, d_tab, r.b, r.k1, r.k2, (u32)n, d, d_tt, tt_mask_word(), r.acc, d_total, d_ttstats,
                         stream) != hipSuccess)
         return fail(PF_EDEVICE);
     st.kernel_launches++;
@@ -249,7 +400,7 @@ pf_status Engine::perft_levels_tt(const pf_pos48* pos, int n, int d, u64* out) {
     ru.acc = r.acc;
     ru.n = (u32)n;
     ru.depth = d;
-    if (launch_upsweep(ru, d_tt, tt_buckets - 1, d_ttstats + 2, stream) != hipSuccess) return fail(PF_EDEVICE);
+    if (launch_upsweep(ru, d_tt, tt_mask_word(), d_ttstats + 2, stream) != hipSuccess) return fail(PF_EDEVICE);
     unsigned long long ts[4];
     if (hipMemcpyAsync(out, r.acc, (size_t)n * 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
         hipMemcpyAsync(ts, d_ttstats, sizeof(ts), hipMemcpyDeviceToHost, stream) != hipSuccess ||

@@ -78,8 +78,14 @@ hipError_t launch_persistent(int backend, int variant, const DevTables& dt, cons
                              hipStream_t s);
 // The same batch as one launch per level on stream s (levels sized and placed
 // on the device; no host round trip): the root count, then depth - 1 level
-// launches of grid[kWrite] / grid[kLeaf] workgroups.
-int stream_grid(int backend, int mode, int n_cu);
-hipError_t launch_stream(int backend, const DevTables& dt, const PersistArgs& a, const int grid[2], hipStream_t s);
+// launches of grid[kWrite] / grid[kLeaf] workgroups of the level kernel's
+// form variant[kWrite] / variant[kLeaf] (stream.hip: tile shape * 3 + waves
+// per SIMD target; the byte backend has one form).
+constexpr int kStreamVariants = 6;
+constexpr int kStreamLeafDefault = 5;   // 256 x 4096 tile, five waves per SIMD
+constexpr int kStreamWriteDefault = 3;  // 256 x 4096 tile, no waves target
+int stream_grid(int backend, int mode, int variant, int n_cu);
+hipError_t launch_stream(int backend, const DevTables& dt, const PersistArgs& a, const int grid[2],
+                         const int variant[2], hipStream_t s);
 
 }  // namespace pf

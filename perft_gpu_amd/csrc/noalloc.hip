@@ -86,7 +86,8 @@ pf_status Engine::perft_noalloc(const pf_pos48* dpos, int n, int d, u64* dout) {
     p.ctl = ctl;
     p.sums = out;
     if (timing && hipEventRecord(ev0, stream) != hipSuccess) return PF_EDEVICE;
-    if (launch_stream(backend, DevTables{d_tab}, p, sgrid[backend], stream) != hipSuccess) return PF_EDEVICE;
+    const int var[2] = {tune[backend].s_leaf, tune[backend].s_write};
+    if (launch_stream(backend, DevTables{d_tab}, p, sgrid[backend], var, stream) != hipSuccess) return PF_EDEVICE;
     if (timing && hipEventRecord(ev1, stream) != hipSuccess) return PF_EDEVICE;
     st.kernel_launches += (u64)d + 2;  // clear, conversion, the root count and d - 1 levels
     st.levels_run += (u64)(d - 1);

@@ -62,6 +62,9 @@ pf_status Server::start() {
     if (hipSetDevice(e_.device) != hipSuccess) return PF_EDEVICE;
     if (hipHostMalloc(&h_tab_, sizeof(StepTable), hipHostMallocDefault) != hipSuccess) return PF_ENOMEM;
     if (hipHostMalloc(&h_acc_, sizeof(u64) * max_jobs_, hipHostMallocDefault) != hipSuccess) return PF_ENOMEM;
+    if (hipHostMalloc(&h_stats_, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        return PF_ENOMEM;
+    memset(h_stats_, 0, 4 * sizeof(unsigned long long));
     if (hipMalloc(&d_tab_, sizeof(StepTable)) != hipSuccess) return PF_ENOMEM;
     const int n = max_jobs_;
     if (hipMalloc(&root_b_, sizeof(Board) * n) != hipSuccess || hipMalloc(&root_w_, 4 * n) != hipSuccess ||
@@ -69,30 +72,57 @@ pf_status Server::start() {
         hipMalloc(&root_p_, 4 * n) != hipSuccess || hipMalloc(&slot_acc_, 8 * n) != hipSuccess)
         return PF_ENOMEM;
     hipMemset(root_p_, 0, 4 * n);
+    if (e_.d_ttstats && hipMemset(e_.d_ttstats, 0, 8 * sizeof(unsigned long long)) != hipSuccess) return PF_EDEVICE;
     memset(h_tab_, 0, sizeof(StepTable));
     jobs_.assign((size_t)n, Job{});
+    seg_tmp_.assign((size_t)kMaxSegs, TTArgs{});
     free_slots_.clear();
     for (int i = n - 1; i >= 0; i--) free_slots_.push_back(i);
     grid_ = step_grid(e_.backend, e_.n_cu);
+    leaf_grid_ = step_max_bpc(e_.backend, 1) * e_.n_cu;
     up_grid_ = e_.n_cu * 4;
-    pf_status s = build_graph();
-    if (s != PF_OK) return s;
     stop_ = false;
-    running_ = true;
-    th_ = std::thread([this] { run(); });
+    // the serving thread captures the step graph itself and then replays it,
+    // so capture and every replay happen on one thread; start() returns the
+    // capture's outcome
+    {
+        std::unique_lock<std::mutex> g(mu_);
+        graph_status_ = 1;
+        th_ = std::thread([this] { run(); });
+        cv_.wait(g, [&] { return graph_status_ != 1; });
+        if (graph_status_ != PF_OK) {
+            const pf_status s = (pf_status)graph_status_;
+            g.unlock();
+            th_.join();
+            return s;
+        }
+        running_ = true;  // under the lock that submit() reads it with
+    }
     return PF_OK;
 }
 
 pf_status Server::build_graph() {
     hipStream_t st = e_.stream;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return PF_EDEVICE;
-    hipMemcpyAsync(d_tab_, h_tab_, sizeof(StepTable), hipMemcpyHostToDevice, st);
-    launch_step(e_.backend, e_.d_tab, e_.d_z, d_tab_, e_.d_tt, e_.tt_buckets ? e_.tt_buckets - 1 : 0, grid_, up_grid_,
-                st);
-    hipMemcpyAsync(&h_tab_->out[0], &d_tab_->out[0], sizeof(h_tab_->out) + sizeof(h_tab_->rout),
-                   hipMemcpyDeviceToHost, st);
-    hipMemcpyAsync(h_acc_, slot_acc_, sizeof(u64) * max_jobs_, hipMemcpyDeviceToHost, st);
-    if (hipStreamEndCapture(st, &graph_) != hipSuccess) return PF_EDEVICE;
+    hipError_t e = hipMemcpyAsync(d_tab_, h_tab_, sizeof(StepTable), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = launch_step(e_.backend, e_.d_tab, e_.d_z, d_tab_, e_.d_tt, e_.tt_buckets ? e_.tt_mask_word() : 0, grid_,
+                        leaf_grid_, up_grid_, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&h_tab_->out[0], &d_tab_->out[0], sizeof(h_tab_->out) + sizeof(h_tab_->rout),
+                           hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_acc_, slot_acc_, sizeof(u64) * max_jobs_, hipMemcpyDeviceToHost, st);
+    // the table's probe and hit counters (expand segments), for pf_get_stats
+    if (e == hipSuccess && e_.d_ttstats)
+        e = hipMemcpyAsync(h_stats_, e_.d_ttstats, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+    // the capture is ended whatever happened inside it, so the stream leaves capture mode
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess || ee != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return PF_EDEVICE;
+    }
+    graph_ = g;
     if (hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0) != hipSuccess) return PF_EDEVICE;
     return PF_OK;
 }
@@ -105,6 +135,7 @@ void Server::stop() {
         }
         cv_.notify_all();
         if (th_.joinable()) th_.join();
+        std::lock_guard<std::mutex> g(mu_);
         running_ = false;
     }
     if (exec_) hipGraphExecDestroy(exec_);
@@ -120,6 +151,8 @@ void Server::stop() {
     hipFree(slot_acc_);
     hipHostFree(h_tab_);
     hipHostFree(h_acc_);
+    hipHostFree(h_stats_);
+    h_stats_ = nullptr;
     d_tab_ = nullptr; root_b_ = nullptr; root_w_ = nullptr; root_k1_ = nullptr; root_k2_ = nullptr;
     root_p_ = nullptr; slot_acc_ = nullptr; h_tab_ = nullptr; h_acc_ = nullptr;
 }
@@ -200,12 +233,14 @@ int Server::poll(pf_result* out, int max) {
     return n;
 }
 
-void Server::counters(u64* steps, u64* segs, double* busy, u64* dups) {
+void Server::counters(u64* steps, u64* segs, double* busy, u64* dups, u64* probes, u64* hits) {
     std::lock_guard<std::mutex> g(mu_);
     *steps = steps_;
     *segs = segs_;
     *busy = busy_;
     *dups = dups_;
+    *probes = probes_ + root_probes_;
+    *hits = hits_ + root_hits_;
 }
 
 void Server::finish(Job& j, u64 nodes, int status) {
@@ -359,7 +394,8 @@ bool Server::plan(Job& j) {
             a.in_base = (u32)f.off;
             a.in_acc = f.lv.acc + f.off;
             a.tt = e_.d_tt;
-            a.tt_mask = e_.tt_buckets ? e_.tt_buckets - 1 : 0;
+            a.tt_mask = e_.tt_buckets ? e_.tt_mask_word() : 0;
+            a.ep_always = e_.exact_ep ? 0 : 1;
             a.child_depth = r - 1;
             a.stats = e_.d_ttstats;
             const u32 si = t.nseg;
@@ -378,20 +414,24 @@ bool Server::plan(Job& j) {
                 // children at ply L+1 get a duplicate table and list (24 + 8 B
                 // per child), which live with their level
                 const bool dd = e_.dedup && e_.d_tt && f.L + 1 >= kDupMinPly;  // keys exist only with a table
-                const u64 node_b = kTNode + (dd ? 32 : 0);
+                // a level that the leaf pass will finish (remaining depth 2) is
+                // never chunked: it needs no block sums and nothing below it, so
+                // it may take all the free memory; others keep 4 B of block sum
+                // per 1024 children (charged as 1 B per child) and leave a quarter
+                const bool leaf_next = r - 1 == 2;
+                const u64 node_b = kTNode + (dd ? 32 : 0) + (leaf_next ? 0 : 1);
                 const u64 free_b = j.cap - j.top;
                 u64 cap;
-                // a level that the leaf pass will finish needs nothing below it:
-                // it may take all the free memory; others leave a quarter
-                const u64 room = (r - 1 == 2) ? free_b : free_b / 2 + free_b / 4;
-                if (!f.chunking && f.n_next * node_b + 10 * kAl <= room) {
+                const u64 room = leaf_next ? free_b : free_b / 2 + free_b / 4;
+                if (!f.chunking && f.n_next * node_b + 11 * kAl <= room) {
                     cap = f.n_next;
                 } else {
                     if (!f.chunking) {
                         f.chunking = true;
                         f.end = f.off + f.m;
                     }
-                    const u64 budget = (r - 1 == 2) ? free_b - std::min<u64>(free_b, 16 * kAl) : free_b / 2;
+                    const u64 budget = leaf_next ? free_b - std::min<u64>(free_b, 16 * kAl)
+                                                 : free_b / 2 - std::min<u64>(free_b / 2, 2 * kAl);
                     if (!chunk(j, f, budget / node_b, &cap)) return false;
                     a.n_in = (u32)f.m;
                 }
@@ -400,10 +440,12 @@ bool Server::plan(Job& j) {
                     j.top = f.mark;
                     return false;  // cannot happen with the half-free rule; wait
                 }
-                f.nx.bsum = reinterpret_cast<u32*>(take(j, ((cap + kBsumBlock - 1) >> kBsumShift) * 4));
-                if (!f.nx.bsum) {
-                    j.top = f.mark;
-                    return false;
+                if (!leaf_next) {
+                    f.nx.bsum = reinterpret_cast<u32*>(take(j, ((cap + kBsumBlock - 1) >> kBsumShift) * 4));
+                    if (!f.nx.bsum) {
+                        j.top = f.mark;
+                        return false;
+                    }
                 }
                 if (dd && cap >= 2) {
                     const u64 dn = dup_table_slots(cap);
@@ -434,7 +476,7 @@ bool Server::plan(Job& j) {
                 a.out_acc = f.nx.acc;
                 a.out_cap = cap;
                 a.out_bsum = f.nx.bsum;
-                a.bsum_n = (u32)((cap + kBsumBlock - 1) >> kBsumShift);
+                a.bsum_n = f.nx.bsum ? (u32)((cap + kBsumBlock - 1) >> kBsumShift) : 0u;
                 a.total = &d_tab_->out[si].total;
                 f.state = ST_WRITE;
             }
@@ -468,7 +510,13 @@ bool Server::plan(Job& j) {
 }
 
 void Server::run() {
-    hipSetDevice(e_.device);
+    {
+        const pf_status gs = hipSetDevice(e_.device) == hipSuccess ? build_graph() : PF_EDEVICE;
+        std::lock_guard<std::mutex> g(mu_);
+        graph_status_ = gs;
+    }
+    cv_.notify_all();
+    if (graph_status_ != PF_OK) return;
     std::vector<int> active;
     std::vector<int> admitted;
     for (;;) {
@@ -565,6 +613,19 @@ void Server::run() {
             active.clear();
             continue;
         }
+        // write segments first, then leaf segments (two launches); a segment
+        // keeps its outcome slot (its out pointers), so jobs find their
+        // outcome where they planned it
+        {
+            u32 nw = 0;
+            for (u32 i = 0; i < t.nseg; i++)
+                if (t.seg[i].mode == kWrite) seg_tmp_[nw++] = t.seg[i];
+            t.nseg_write = nw;
+            for (u32 i = 0; i < t.nseg; i++)
+                if (t.seg[i].mode != kWrite) seg_tmp_[nw++] = t.seg[i];
+            std::copy(seg_tmp_.begin(), seg_tmp_.begin() + t.nseg, t.seg);
+            t.tile_ctr2 = 0;
+        }
         // tile prefix over expand segments
         u32 acc_t = 0;
         for (u32 i = 0; i < t.nseg; i++) {
@@ -594,6 +655,15 @@ void Server::run() {
             steps_++;
             segs_ += t.nseg + t.nup + t.nroot;
             busy_ += t1 - t0;
+            if (ok && e_.d_tt) {
+                probes_ = h_stats_[0];
+                hits_ = h_stats_[1];
+                for (u32 i = 0; i < t.nroot; i++) {
+                    if (t.root[i].depth < kMinTTDepth) continue;
+                    root_probes_++;
+                    root_hits_ += t.rout[i].hit;
+                }
+            }
         }
         if (!ok) {
             for (int slot : active)

@@ -21,7 +21,9 @@ class Server {
     void stop();
     pf_status submit(const pf_pos48& pos, int depth, u64 tag);
     int poll(pf_result* out, int max);
-    void counters(u64* steps, u64* segs, double* busy_s, u64* dups);
+    // probes / hits: table probes and hits of the expand segments and of the
+    // roots (remaining depth >= 3) since the server started
+    void counters(u64* steps, u64* segs, double* busy_s, u64* dups, u64* probes, u64* hits);
 
   private:
     struct Frame {
@@ -70,6 +72,7 @@ class Server {
     std::mutex mu_;
     std::condition_variable cv_;
     bool stop_ = false, running_ = false;
+    int graph_status_ = 0;  // the serving thread's graph capture: 1 pending, else a pf_status
     std::deque<Job> queue_;        // submitted, not yet admitted
     std::vector<pf_result> done_;  // finished, not yet polled
     std::vector<Job> jobs_;        // indexed by slot
@@ -88,7 +91,10 @@ class Server {
     u32* root_p_ = nullptr;
     unsigned long long* slot_acc_ = nullptr;
     unsigned long long* h_acc_ = nullptr;
-    int grid_ = 0, up_grid_ = 0;
+    unsigned long long* h_stats_ = nullptr;  // the table counters, copied back by every step
+    u64 probes_ = 0, hits_ = 0, root_probes_ = 0, root_hits_ = 0;
+    int grid_ = 0, leaf_grid_ = 0, up_grid_ = 0;
+    std::vector<TTArgs> seg_tmp_;  // reordering buffer of a step's segments
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
     u64 steps_ = 0, segs_ = 0, dups_ = 0;

@@ -26,6 +26,29 @@ namespace pf {
 constexpr u64 kKey2Salt = 0xD1B54A32D192ED03ULL;
 constexpr u64 kCountMask = (1ULL << 56) - 1;
 
+// A table's "mask word": bits 0-55 are buckets - 1; bits 56-59 hold k > 0
+// when the table is laid out as 2^k equal regions, one per remaining depth
+// modulo 2^k (the reference's one table per depth, launcher.h:62-126),
+// instead of one table shared by every depth (k = 0, the default: the depth
+// is part of each entry, so entries of all depths compete for all buckets).
+PF_HD u64 tt_bucket(u64 k1, int depth, u64 mw) {
+    const u64 m = mw & kCountMask;
+    const int k = (int)(mw >> 56);
+    if (!k) return k1 & m;
+    const u64 sub = m >> k;  // buckets per region - 1
+    return (u64)(depth & ((1 << k) - 1)) * (sub + 1) + (k1 & sub);
+}
+
+// The reference CPU generator's en-passant flagging (generateBoards,
+// MoveGeneratorBitboard.h:1575-1582): the file is set after every double
+// push, whether or not an enemy pawn can capture.  make_move flags it only
+// when one can (EXACT_EN_PASSENT_FLAGGING, :3528-3547); counts are the same
+// either way, keys (and so table and duplicate matches) are not.
+PF_HD Board ep_on_every_double_push(Board c, u32 m) {
+    if (((m >> 12) & 15) == kFlDouble) c.pawns = (c.pawns & ~0xF0ULL) | ((u64)((m & 7) + 1) << 4);
+    return c;
+}
+
 struct ZKeys {
     u64 piece[2][6][64];
     u64 castle16[16];  // XOR of castle[] for each absolute rights nibble

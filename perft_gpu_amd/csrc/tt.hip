@@ -70,7 +70,7 @@ __device__ __forceinline__ void wave_run_add(unsigned long long* p, unsigned lon
 }
 
 __device__ __forceinline__ bool tt_probe(const uint4* tt, u64 mask, u64 k1, u64 k2, int depth, u64* val) {
-    const uint4* bkt = tt + 2 * (k1 & mask);
+    const uint4* bkt = tt + 2 * tt_bucket(k1, depth, mask);
 #pragma unroll
     for (int s = 0; s < 2; s++) {
         const uint4 e = bkt[s];
@@ -85,7 +85,7 @@ __device__ __forceinline__ bool tt_probe(const uint4* tt, u64 mask, u64 k1, u64 
 
 __device__ __forceinline__ void tt_store(uint4* tt, u64 mask, u64 k1, u64 k2, int depth, u64 val) {
     if (val > kCountMask) return;
-    uint4* bkt = tt + 2 * (k1 & mask);
+    uint4* bkt = tt + 2 * tt_bucket(k1, depth, mask);
     const uint4 e0 = bkt[0];
     const u64 o1 = (u64)e0.z | ((u64)e0.w << 32);
     const int od = (int)(o1 >> 56);
@@ -192,7 +192,8 @@ __device__ __forceinline__ void write_window(const SL& sl, const TTArgs& a, cons
             const u32 e = s.mv[j];
             const u32 par = (e >> 16) & (kTTP - 1), mvv = e & 0xFFFFu;
             const Board pb = s.b[par];
-            const Board c = make_move(pb, mvv);
+            Board c = make_move(pb, mvv);
+            if (a.ep_always) c = ep_on_every_double_push(c, mvv);
             u64 ck1 = 0, ck2 = 0;
             if (a.tt) {
                 const int black = (int)(s.root[par] >> 31);
@@ -260,6 +261,7 @@ __device__ __forceinline__ void write_window(const SL& sl, const TTArgs& a, cons
         if (keep || isdup) {
             const Board pb = s.b[par];
             c = make_move(pb, mvv);
+            if (a.ep_always) c = ep_on_every_double_push(c, mvv);
             if (a.tt) {
                 const int black = (int)(s.root[par] >> 31);
                 ck1 = zobrist_child(zt->k[0], s.k1[par], pb, mvv, black, c);
@@ -511,20 +513,25 @@ __device__ __forceinline__ TTArgs uniform_args(const TTArgs& src) {
 // Tiles of all expand segments form one queue (StepTable::tile_prefix); a
 // block copies its segment's arguments to LDS and runs the tile.  Segments of
 // different jobs touch disjoint memory except the lockless table.
-template <int B>
-__global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(kTTWaves))) void k_step(const SliderTables* tabs, const ZTables* zt, StepTable* t) {
+// W: the waves per SIMD it is built for; M: kWrite runs the table's write
+// segments [0, nseg_write), kLeaf its leaf segments [nseg_write, nseg), each
+// without the other mode's code (as in k_expand_tt); -1 runs every segment.
+template <int B, int W = kTTWaves, int M = -1>
+__global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void k_step(const SliderTables* tabs, const ZTables* zt, StepTable* t) {
     __shared__ TTLds s;
     __shared__ TTArgs sa;
     __shared__ u32 s_tile, s_seg;
     const auto sl = SlOfT<B>::make(tabs);
-    const u32 nseg = t->nseg, total = t->tile_prefix[nseg];
+    const u32 s_lo = (M == kLeaf) ? t->nseg_write : 0u, s_hi = (M == kWrite) ? t->nseg_write : t->nseg;
+    const u32 first = t->tile_prefix[s_lo], total = t->tile_prefix[s_hi];
+    u32* const ctr = (M == kLeaf) ? &t->tile_ctr2 : &t->tile_ctr;
     u32 cur = 0xFFFFFFFFu;
     for (;;) {
         if (threadIdx.x == 0) {
-            const u32 g = atomicAdd(&t->tile_ctr, 1u);
-            u32 seg = 0;
+            const u32 g = first + atomicAdd(ctr, 1u);
+            u32 seg = s_lo;
             if (g < total) {
-                u32 lo = 0, hi = nseg;  // largest seg with tile_prefix[seg] <= g
+                u32 lo = s_lo, hi = s_hi;  // largest seg with tile_prefix[seg] <= g
                 while (hi - lo > 1) {
                     const u32 mid = (lo + hi) >> 1;
                     if (t->tile_prefix[mid] <= g) lo = mid; else hi = mid;
@@ -542,7 +549,9 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(kTTWaves))
             cur = seg;
         }
         __syncthreads();
-        tt_tile(sl, uniform_args(sa), g - t->tile_prefix[seg], zt, s);
+        TTArgs a = uniform_args(sa);
+        if (M >= 0) a.mode = M;
+        tt_tile(sl, a, g - t->tile_prefix[seg], zt, s);
     }
 }
 
@@ -551,6 +560,7 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(kTTWaves))
 __global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 mask, int round) {
     const u32 s0 = t->up_round[round], s1 = t->up_round[round + 1];
     const u32 first = t->up_prefix[s0], last = t->up_prefix[s1];
+    unsigned long long stores = 0;
     // the loop is uniform per wave (wave_run_add needs every lane)
     for (u32 w = first + blockIdx.x * 256 + (threadIdx.x & ~63u); w < last; w += gridDim.x * 256) {
         const u32 i = w + (threadIdx.x & 63);
@@ -562,9 +572,13 @@ __global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 ma
                 const u32 mid = (lo + hi) >> 1;
                 if (t->up_prefix[mid] <= i) lo = mid; else hi = mid;
             }
-            upsweep_item(t->up[lo], i - t->up_prefix[lo], tt, mask, &dst, &v);
+            stores += upsweep_item(t->up[lo], i - t->up_prefix[lo], tt, mask, &dst, &v);
         }
         wave_run_add(dst, v);
+    }
+    if (t->stores) {
+        stores = t_wave_sum64(stores);
+        if ((threadIdx.x & 63) == 0 && stores) atomicAdd(t->stores, stores);
     }
 }
 
@@ -607,32 +621,297 @@ __global__ __launch_bounds__(256) void k_pre_step(const SliderTables* tabs, Step
     }
 }
 
-int step_grid(int backend, int n_cu) {
-    const void* f = backend == kKS ? (const void*)&k_step<kKS> : backend == kPlain ? (const void*)&k_step<kPlain>
-                                                                                   : (const void*)&k_step<kFancy>;
+// Most resident workgroups per CU of the planned pass's step kernels.
+int step_max_bpc(int backend, int leaf) {
+    const void* f;
+    if (leaf)
+        f = backend == kKS ? (const void*)&k_step<kKS, kTTLeafWaves, kLeaf>
+            : backend == kPlain ? (const void*)&k_step<kPlain, kTTLeafWaves, kLeaf>
+            : backend == kFancy ? (const void*)&k_step<kFancy, kTTLeafWaves, kLeaf>
+                                : (const void*)&k_step<kByte, kTTLeafWaves, kLeaf>;
+    else
+        f = backend == kKS ? (const void*)&k_step<kKS, kTTWaves, kWrite>
+            : backend == kPlain ? (const void*)&k_step<kPlain, kTTWaves, kWrite>
+            : backend == kFancy ? (const void*)&k_step<kFancy, kTTWaves, kWrite>
+                                : (const void*)&k_step<kByte, kTTWaves, kWrite>;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kTWG, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-    return per_cu * n_cu;
+    return per_cu;
 }
+
+int step_grid(int backend, int n_cu) { return step_max_bpc(backend, 0) * n_cu; }
 
 // One step: start (roots, fills), expand/leaf segments, kUpRounds upsweep
 // rounds.  up_grid * 256 must cover kMaxRoots (the roots are handled by the
 // first threads of the start kernel).
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
-                       int grid, int up_grid, hipStream_t s) {
+                       int grid, int leaf_grid, int up_grid, hipStream_t s) {
     if (up_grid * 256 < kMaxRoots) return hipErrorInvalidValue;
     switch (backend) {
     case kKS: hipLaunchKernelGGL(k_pre_step<kKS>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask); break;
     case kPlain: hipLaunchKernelGGL(k_pre_step<kPlain>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask); break;
-    default: hipLaunchKernelGGL(k_pre_step<kFancy>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask); break;
+    case kFancy: hipLaunchKernelGGL(k_pre_step<kFancy>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask); break;
+    default: hipLaunchKernelGGL(k_pre_step<kByte>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask); break;
     }
+    // the write segments, then the leaf segments (five-wave leaf build)
     switch (backend) {
-    case kKS: hipLaunchKernelGGL(k_step<kKS>, dim3(grid), dim3(kTWG), 0, s, tabs, zt, t); break;
-    case kPlain: hipLaunchKernelGGL(k_step<kPlain>, dim3(grid), dim3(kTWG), 0, s, tabs, zt, t); break;
-    default: hipLaunchKernelGGL(k_step<kFancy>, dim3(grid), dim3(kTWG), 0, s, tabs, zt, t); break;
+    case kKS:
+        hipLaunchKernelGGL((k_step<kKS, kTTWaves, kWrite>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+        hipLaunchKernelGGL((k_step<kKS, kTTLeafWaves, kLeaf>), dim3(leaf_grid), dim3(kTWG), 0, s, tabs, zt, t);
+        break;
+    case kPlain:
+        hipLaunchKernelGGL((k_step<kPlain, kTTWaves, kWrite>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+        hipLaunchKernelGGL((k_step<kPlain, kTTLeafWaves, kLeaf>), dim3(leaf_grid), dim3(kTWG), 0, s, tabs, zt, t);
+        break;
+    case kFancy:
+        hipLaunchKernelGGL((k_step<kFancy, kTTWaves, kWrite>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+        hipLaunchKernelGGL((k_step<kFancy, kTTLeafWaves, kLeaf>), dim3(leaf_grid), dim3(kTWG), 0, s, tabs, zt, t);
+        break;
+    default:
+        hipLaunchKernelGGL((k_step<kByte, kTTWaves, kWrite>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+        hipLaunchKernelGGL((k_step<kByte, kTTLeafWaves, kLeaf>), dim3(leaf_grid), dim3(kTWG), 0, s, tabs, zt, t);
+        break;
     }
     for (int r = 0; r < kUpRounds; r++) hipLaunchKernelGGL(k_up_step, dim3(up_grid), dim3(256), 0, s, t, tt, mask, r);
     return hipGetLastError();
+}
+
+// ---- device-planned table pass -------------------------------------------------
+// One-block planners between the launches of a captured pass (tt_kernels.h):
+// they read the counts the previous launch left in the step table, place the
+// next level in the pass's memory and write the next launch's segment.  A
+// level that does not fit sets st.err; every later planner then leaves its
+// launch empty, so the pass ends without an effect outside its memory.
+
+// Takes `bytes` (rounded up to 256) of the pass's memory; on overflow sets err.
+__device__ __forceinline__ char* plan_take(TTPlanCtl* c, u64 bytes) {
+    const u64 b = (bytes + 255) & ~255ULL;
+    if (c->st.err || c->st.peak + b > c->p.arena_bytes) {
+        c->st.err = 2;
+        return nullptr;
+    }
+    char* r = c->p.arena + c->st.peak;
+    c->st.peak += b;
+    return r;
+}
+
+// Empties the step table's launch descriptions (an empty step does nothing).
+__device__ __forceinline__ void plan_empty(StepTable* t) {
+    t->nseg = 0;
+    t->nseg_write = 0;
+    t->nup = 0;
+    t->nroot = 0;
+    t->tile_ctr = 0;
+    t->tile_ctr2 = 0;
+    t->tile_prefix[0] = 0;
+    t->tile_prefix[1] = 0;
+    for (int r = 0; r <= kUpRounds; r++) t->up_round[r] = 0;
+    t->up_prefix[0] = 0;
+    t->out[0] = SegOut{};
+}
+
+// Level L of the pass as written by the step that just ran.
+__device__ __forceinline__ void plan_record(TTPlanCtl* c, const StepTable* t, int L) {
+    const SegOut& o = t->out[0];
+    if (o.err) c->st.err = 2;
+    TTLevelRec& lv = c->lv[L];
+    lv.n = o.out_count;
+    lv.ndup = o.ndup;
+    lv.next = o.total;
+    c->st.frontier += o.out_count;
+    c->st.dups += o.ndup;
+}
+
+// Before the write of level L + 1 from level L.  At L = 0 it also sets up the
+// pass: the parents (the caller's level, read only) with accumulators of the
+// pass's own, and the children total from the parents' stored move counts.
+__global__ __launch_bounds__(256) void k_plan_write(TTPlanCtl* c, StepTable* t, int L, uint4* tt, u64 mask,
+                                                    unsigned long long* ttstats) {
+    __shared__ unsigned long long red[4];
+    if (L == 0) {
+        const TTLevelRec& top = c->p.top;
+        unsigned long long v = 0;
+        for (u64 i = threadIdx.x; i < top.n; i += 256) v += (u64)meta_count(top.b[i]);
+        v = t_wave_sum64(v);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (L == 0) {
+        c->st = TTPlanStatus{};
+        c->lv[0] = c->p.top;
+        c->lv[0].next = red[0] + red[1] + red[2] + red[3];
+        c->lv[0].acc = reinterpret_cast<unsigned long long*>(plan_take(c, c->p.top.n * 8));
+    } else if (!c->st.err) {
+        plan_record(c, t, L);
+    }
+    plan_empty(t);
+    if (c->st.err) return;
+    const TTLevelRec& cur = c->lv[L];
+    const u64 cap = cur.next;
+    if (cap > 0xFFFFFFFFull || cur.n > 0xFFFFFFFFull) {
+        c->st.err = 2;
+        return;
+    }
+    c->st.made += cap;
+    TTLevelRec nx = {};
+    nx.b = reinterpret_cast<Board*>(plan_take(c, cap * sizeof(Board)));
+    nx.root = reinterpret_cast<u32*>(plan_take(c, cap * 4));
+    nx.k1 = reinterpret_cast<u64*>(plan_take(c, cap * 8));
+    nx.k2 = reinterpret_cast<u64*>(plan_take(c, cap * 8));
+    nx.parent = reinterpret_cast<u32*>(plan_take(c, cap * 4));
+    nx.acc = reinterpret_cast<unsigned long long*>(plan_take(c, cap * 8));
+    const bool dd = c->p.dedup && c->p.ply + L + 1 >= kDupMinPly && cap >= 2;
+    if (dd) {
+        nx.dup_n = dup_table_slots(cap);
+        nx.dup = reinterpret_cast<DupEntry*>(plan_take(c, nx.dup_n * sizeof(DupEntry)));
+        nx.dpar = reinterpret_cast<u32*>(plan_take(c, cap * 4));
+        nx.dslot = reinterpret_cast<u32*>(plan_take(c, cap * 4));
+    }
+    c->lv[L + 1] = nx;
+    if (c->st.err) return;
+    TTArgs a = {};
+    a.in = cur.b;
+    a.in_root = cur.root;
+    a.in_k1 = cur.k1;
+    a.in_k2 = cur.k2;
+    a.n_in = (u32)cur.n;
+    a.in_base = 0;
+    a.in_acc = cur.acc;
+    a.out = nx.b;
+    a.out_root = nx.root;
+    a.out_k1 = nx.k1;
+    a.out_k2 = nx.k2;
+    a.out_parent = nx.parent;
+    a.out_acc = nx.acc;
+    a.out_cap = cap;
+    a.out_count = &t->out[0].out_count;
+    a.total = &t->out[0].total;
+    a.tt = tt;
+    a.tt_mask = mask;
+    a.child_depth = c->p.depth - (L + 1);
+    a.mode = kWrite;
+    a.ep_always = c->p.ep_always;
+    a.err = &t->out[0].err;
+    a.stats = ttstats;
+    if (dd) {
+        a.dup = nx.dup;
+        a.dup_n = nx.dup_n;
+        a.dup_count = &t->out[0].ndup;
+        a.dup_par = nx.dpar;
+        a.dup_slot = nx.dslot;
+    }
+    t->seg[0] = a;
+    t->tile_prefix[1] = (u32)((cur.n + kTTP - 1) / kTTP);
+    t->nseg = 1;
+    t->nseg_write = 1;
+}
+
+// Empties level L + 1's duplicate table and, at L = 0, the pass's accumulators.
+__global__ __launch_bounds__(256) void k_plan_clear(TTPlanCtl* c, int L) {
+    if (c->st.err) return;
+    const u64 stride = (u64)gridDim.x * 256, i0 = (u64)blockIdx.x * 256 + threadIdx.x;
+    const TTLevelRec& nx = c->lv[L + 1];
+    for (u64 k = i0; k < nx.dup_n; k += stride) nx.dup[k] = DupEntry{0, 0};
+    if (L == 0)
+        for (u64 k = i0; k < c->lv[0].n; k += stride) c->lv[0].acc[k] = 0;
+}
+
+// Before the leaf pass over level L (remaining depth 2).
+__global__ __launch_bounds__(64) void k_plan_leaf(TTPlanCtl* c, StepTable* t, int L,
+                                                  unsigned long long* ttstats) {
+    if (threadIdx.x != 0) return;
+    if (!c->st.err) plan_record(c, t, L);
+    plan_empty(t);
+    if (c->st.err) return;
+    const TTLevelRec& cur = c->lv[L];
+    c->st.made += cur.next;
+    TTArgs a = {};
+    a.in = cur.b;
+    a.in_root = cur.root;
+    a.in_k1 = cur.k1;
+    a.in_k2 = cur.k2;
+    a.n_in = (u32)cur.n;
+    a.in_base = 0;
+    a.in_acc = cur.acc;
+    a.out_count = &t->out[0].out_count;
+    a.child_depth = c->p.depth - (L + 1);
+    a.mode = kLeaf;
+    a.err = &t->out[0].err;
+    a.stats = ttstats;
+    t->seg[0] = a;
+    t->tile_prefix[1] = (u32)((cur.n + kTTP - 1) / kTTP);
+    t->nseg = 1;
+}
+
+// Before the upsweep of level L + 1 into level L.
+__global__ __launch_bounds__(64) void k_plan_up(TTPlanCtl* c, StepTable* t, int L, unsigned long long* ttstats) {
+    if (threadIdx.x != 0) return;
+    plan_empty(t);
+    if (c->st.err) return;
+    const TTLevelRec& ch = c->lv[L + 1];
+    UpSeg u = {};
+    u.parent = ch.parent;
+    u.k1 = ch.k1;
+    u.k2 = ch.k2;
+    u.acc = ch.acc;
+    u.parent_acc = c->lv[L].acc;
+    u.dup = ch.dup;
+    u.dup_par = ch.dpar;
+    u.dup_slot = ch.dslot;
+    u.n = (u32)ch.n;
+    u.ndup = (u32)ch.ndup;
+    u.depth = c->p.depth - (L + 1);
+    t->up[0] = u;
+    t->nup = 1;
+    for (int r = 1; r <= kUpRounds; r++) t->up_round[r] = 1;
+    t->up_prefix[1] = u.n + u.ndup;
+    t->stores = ttstats ? ttstats + 2 : nullptr;
+}
+
+// The pass's counts into the caller's accumulators, if nothing overflowed.
+__global__ __launch_bounds__(256) void k_plan_commit(TTPlanCtl* c) {
+    if (c->st.err) return;
+    const u64 stride = (u64)gridDim.x * 256;
+    unsigned long long* dst = c->p.top.acc;
+    const unsigned long long* src = c->lv[0].acc;
+    for (u64 k = (u64)blockIdx.x * 256 + threadIdx.x; k < c->p.top.n; k += stride) dst[k] += src[k];
+}
+
+template <int B>
+static void plan_steps(const SliderTables* tabs, const ZTables* zt, TTPlanCtl* ctl, StepTable* tab, uint4* tt,
+                       u64 mask, unsigned long long* ttstats, int r, int write_grid, int leaf_grid, int up_grid,
+                       hipStream_t s) {
+    for (int L = 0; L <= r - 3; L++) {
+        hipLaunchKernelGGL(k_plan_write, dim3(1), dim3(256), 0, s, ctl, tab, L, tt, mask, ttstats);
+        hipLaunchKernelGGL(k_plan_clear, dim3(up_grid), dim3(256), 0, s, ctl, L);
+        hipLaunchKernelGGL((k_step<B, kTTWaves, kWrite>), dim3(write_grid), dim3(kTWG), 0, s, tabs, zt, tab);
+    }
+    hipLaunchKernelGGL(k_plan_leaf, dim3(1), dim3(64), 0, s, ctl, tab, r - 2, ttstats);
+    hipLaunchKernelGGL((k_step<B, kTTLeafWaves, kLeaf>), dim3(leaf_grid), dim3(kTWG), 0, s, tabs, zt, tab);
+    for (int L = r - 3; L >= 0; L--) {
+        hipLaunchKernelGGL(k_plan_up, dim3(1), dim3(64), 0, s, ctl, tab, L, ttstats);
+        hipLaunchKernelGGL(k_up_step, dim3(up_grid), dim3(256), 0, s, tab, tt, mask, 0);
+    }
+    hipLaunchKernelGGL(k_plan_commit, dim3(up_grid), dim3(256), 0, s, ctl);
+}
+
+hipError_t capture_plan_pass(int backend, const SliderTables* tabs, const ZTables* zt, TTPlanCtl* ctl,
+                             StepTable* tab, const TTPlanParams* h_params, TTPlanStatus* h_status, uint4* tt,
+                             u64 mask, unsigned long long* ttstats, int r, int write_grid, int leaf_grid,
+                             int up_grid, hipStream_t s) {
+    if (r < 3 || r > kPlanMaxLevels - 1) return hipErrorInvalidValue;
+    hipError_t e = hipMemcpyAsync(&ctl->p, h_params, sizeof(TTPlanParams), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    switch (backend) {
+    case kKS: plan_steps<kKS>(tabs, zt, ctl, tab, tt, mask, ttstats, r, write_grid, leaf_grid, up_grid, s); break;
+    case kPlain: plan_steps<kPlain>(tabs, zt, ctl, tab, tt, mask, ttstats, r, write_grid, leaf_grid, up_grid, s); break;
+    case kFancy: plan_steps<kFancy>(tabs, zt, ctl, tab, tt, mask, ttstats, r, write_grid, leaf_grid, up_grid, s); break;
+    default: plan_steps<kByte>(tabs, zt, ctl, tab, tt, mask, ttstats, r, write_grid, leaf_grid, up_grid, s); break;
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(h_status, &ctl->st, sizeof(TTPlanStatus), hipMemcpyDeviceToHost, s);
 }
 
 // ---- launchers ---------------------------------------------------------------
@@ -641,7 +920,8 @@ static void* pick_mode(int backend) {
     switch (backend) {
     case kKS: return reinterpret_cast<void*>(&k_expand_tt<kKS, M>);
     case kPlain: return reinterpret_cast<void*>(&k_expand_tt<kPlain, M>);
-    default: return reinterpret_cast<void*>(&k_expand_tt<kFancy, M>);  // byte: global tables here
+    case kFancy: return reinterpret_cast<void*>(&k_expand_tt<kFancy, M>);
+    default: return reinterpret_cast<void*>(&k_expand_tt<kByte, M>);
     }
 }
 static void* pick(int backend, int mode) { return mode == kLeaf ? pick_mode<kLeaf>(backend) : pick_mode<kWrite>(backend); }
@@ -676,8 +956,11 @@ hipError_t launch_roots_tt(int backend, const SliderTables* tabs, Board* roots, 
     case kPlain:
         hipLaunchKernelGGL(k_roots_tt<kPlain>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total, stats);
         break;
-    default:
+    case kFancy:
         hipLaunchKernelGGL(k_roots_tt<kFancy>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total, stats);
+        break;
+    default:
+        hipLaunchKernelGGL(k_roots_tt<kByte>, g, b, 0, s, tabs, roots, k1, k2, n, depth, tt, mask, acc, total, stats);
         break;
     }
     return hipGetLastError();

@@ -93,7 +93,7 @@ struct Staging {
 }  // namespace
 
 pf_status Engine::tt_save(const char* path) {
-    if (!d_tt || !path) return PF_EINVAL;
+    if (!d_tt || !path || tt_split) return PF_EINVAL;  // the file holds the one-table layout
     if (hipSetDevice(device) != hipSuccess) return PF_EDEVICE;
     Staging g(stream);
     const u64 total = tt_buckets * kBucketBytes, chunk = std::min(total, kChunkBytes);
@@ -130,7 +130,7 @@ pf_status Engine::tt_save(const char* path) {
 }
 
 pf_status Engine::tt_load(const char* path) {
-    if (!d_tt || !path) return PF_EINVAL;
+    if (!d_tt || !path || tt_split) return PF_EINVAL;
     if (hipSetDevice(device) != hipSuccess) return PF_EDEVICE;
     Staging g(stream);
     g.f = fopen(path, "rb");

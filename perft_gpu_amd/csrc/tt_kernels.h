@@ -8,6 +8,9 @@
 
 namespace pf {
 
+// The table kernels' slider backends.  Their tiles fill the LDS, so the
+// byte-lookup backend reads its tables (the same 146 KiB) from global memory
+// through the caches instead of staging them in LDS.
 template <int B>
 using SlOfT = SlOf<B>;
 
@@ -36,7 +39,7 @@ constexpr u32 kBsumBlock = 1u << kBsumShift;
 
 // Slots of the duplicate table of a level of n nodes: linear probing at a load
 // of at most 2/3, indices below 2^32.
-inline u64 dup_table_slots(u64 n) {
+PF_HD u64 dup_table_slots(u64 n) {
     const u64 s = n + n / 2 + 64;
     return s > 0xFFFFFFFFull ? 0xFFFFFFFFull : s;
 }
@@ -64,6 +67,7 @@ struct TTArgs {
     u64 tt_mask;                 // buckets - 1
     int child_depth;             // remaining depth of the children
     int mode;                    // kWrite or kLeaf
+    int ep_always;               // children flag en passant after every double push (keys differ)
     u32* tile_ctr;
     u32* err;
     unsigned long long* stats;   // [0] probes, [1] hits
@@ -124,6 +128,10 @@ constexpr int kUpRounds = 4;
 
 struct StepTable {
     u32 nseg, nup, nroot, tile_ctr;
+    // segments [0, nseg_write) are write levels, [nseg_write, nseg) leaf
+    // passes: each kind runs in a launch of its own (its own tile counter)
+    u32 nseg_write, tile_ctr2;
+    unsigned long long* stores;    // table stores of the upsweeps are counted here (null: not counted)
     u32 tile_prefix[kMaxSegs + 1];
     u32 up_prefix[kMaxUps + 1];    // item prefix over all upsweep segments
     u32 up_round[kUpRounds + 1];   // segments [up_round[r], up_round[r+1]) belong to round r
@@ -134,9 +142,65 @@ struct StepTable {
     RootOut rout[kMaxRoots];
 };
 
+// ---- device-planned table pass ----------------------------------------------
+// A chunk of parents whose whole subtree fits the arena runs as one replay of
+// a captured graph (engine_tt.cpp): every write level is sized and placed on
+// the device by a one-block planner from the counts the previous launch left
+// there, so no level returns to the host (the reference's per-level launch
+// and device sync, perft_bb.h:1071-1135).  Per write level: plan, clear the
+// level's duplicate table, expand; then the leaf pass; then one upsweep per
+// level on the way back up; then the commit.  The pass accumulates the
+// parents' counts in an array of its own and adds them to the caller's only
+// at the commit, and only if no level overflowed: an overflowed pass leaves
+// the caller's accumulators, the arena below it and the table unchanged.
+constexpr int kPlanMaxLevels = 18;
+struct TTLevelRec {
+    Board* b;
+    u32* root;
+    u64* k1;
+    u64* k2;
+    u32* parent;
+    unsigned long long* acc;
+    DupEntry* dup;
+    u64 dup_n;
+    u32* dpar;
+    u32* dslot;
+    u64 n;     // nodes in the level
+    u64 ndup;  // duplicates listed with it
+    u64 next;  // sum of its nodes' move counts: the size of the level below
+};
+struct TTPlanParams {  // the host's part, copied in by the graph's first node
+    TTLevelRec top;    // the chunk's parents (acc: the caller's accumulators)
+    char* arena;       // the pass's memory [arena, arena + arena_bytes)
+    u64 arena_bytes;
+    int depth;         // remaining depth of the parents (>= 3)
+    int ply;           // ply of the parents (duplicate tables from kDupMinPly)
+    int dedup;
+    int ep_always;     // TTArgs::ep_always of the pass's write levels
+};
+struct TTPlanStatus {  // copied back by the graph's last node
+    u64 peak;          // arena bytes the pass used
+    u32 err;           // 2: a level did not fit (the pass had no effect)
+    u32 pad;
+    unsigned long long made, frontier, dups;
+};
+struct TTPlanCtl {
+    TTPlanParams p;
+    TTPlanStatus st;
+    TTLevelRec lv[kPlanMaxLevels];
+};
+// Appends one pass for parents of remaining depth r (3..kPlanMaxLevels) to
+// the stream being captured: params H2D from *h_params, the kernels, status
+// D2H to *h_status.
+hipError_t capture_plan_pass(int backend, const SliderTables* tabs, const ZTables* zt, TTPlanCtl* ctl,
+                             StepTable* tab, const TTPlanParams* h_params, TTPlanStatus* h_status, uint4* tt,
+                             u64 mask, unsigned long long* ttstats, int r, int write_grid, int leaf_grid,
+                             int up_grid, hipStream_t s);
+
 int step_grid(int backend, int n_cu);
+int step_max_bpc(int backend, int leaf);  // the planned pass's write (0) / leaf (1) step kernel
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
-                       int grid, int up_grid, hipStream_t s);
+                       int grid, int leaf_grid, int up_grid, hipStream_t s);
 int expand_tt_max_bpc(int backend);   // most resident workgroups per CU of the TT write-level kernel
 int expand_tt_leaf_bpc(int backend);  // the same for the TT leaf-level kernel
 hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables* zt, const TTArgs& a, int grid,

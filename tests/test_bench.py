@@ -46,10 +46,36 @@ def test_op_model_per_made_node_matches_the_fixture():
 
 
 def test_cpu_c1_reports_the_initial_position_count():
-    """the C1 CPU leg: Kogge-Stone oracle on one thread, median of its runs"""
+    """the C1 CPU leg: Kogge-Stone oracle on one thread and split by root move
+    over several, median / p10 / p90 of the runs"""
     sys.path.insert(0, ROOT)
     import bench
     from perft_gpu_amd import START_FEN
-    r = bench.cpu_c1(START_FEN, 4, 197281, reps=3)
+    r = bench.cpu_c1(START_FEN, 4, 197281, 3, reps=3)
     assert r["counts_match"] and r["cores"] == 1 and r["backend"] == "ks" and r["runs"] == 3
     assert abs(r["value"] - 197281 / r["median_s"]) < 1e-6 * r["value"]
+    assert r["p10_s"] <= r["median_s"] <= r["p90_s"]
+    a = r["all_cores"]
+    assert a["cores"] == 3 and abs(a["value"] - 197281 / a["median_s"]) < 1e-6 * a["value"]
+
+
+def test_host_cpus_reports_the_share_used():
+    sys.path.insert(0, ROOT)
+    import bench
+    h = bench.host_cpus()
+    assert 1 <= h["threads"] <= 16 and h["threads"] <= h["allowed"] and h["nproc"] >= 1
+
+
+def test_cpu_subset_legs_check_against_the_deep_goldens():
+    """the C3 / C5 CPU legs on tiny slices: counts equal to the golden ones
+    (the deep golden file holds the C3 divide and the C5 perft 6 counts)"""
+    sys.path.insert(0, ROOT)
+    import bench
+    from perft_gpu_amd import workloads as W
+    g = bench.load_goldens()
+    if "c5_deep" in g:
+        r = bench.cpu_c5_subset(W.positions(W.C5), g["c5_deep"], 4, n=1)
+        assert r["counts_match_golden"] is True and r["nodes"] == g["c5_deep"]["perft6"][0]
+    assert "c3_deep" in g and g["c3_deep"]["perft8"] == sum(g["c3_deep"]["divide_d8"].values())
+    # the golden divide's root moves are the C3 position's legal moves
+    assert len(g["c3_deep"]["divide_d8"]) == len(g["c3_deep"]["moves"])

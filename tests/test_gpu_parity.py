@@ -157,7 +157,9 @@ def test_autotune_keeps_counts_and_round_trips(pg, oracle, tmp_path):
     text = open(path).read()
     assert text.startswith("perft_gpu_amd tuning 1") and "backend ks persistent_variant" in text
     ks = [l for l in text.splitlines() if l.startswith("backend ks ")][0]
-    assert ks.endswith("single_pass stream") or ks.endswith("single_pass persistent"), ks
+    assert " single_pass stream " in ks or " single_pass persistent " in ks, ks
+    for key in ("stream_leaf", "stream_write", "tt_leaf_blocks_per_cu", "tt_chunk_pct"):
+        assert " %s " % key in ks, ks
     eng = pg.Engine(backend="ks", arena_bytes=2 << 30, tune_file=path)
     try:
         assert np.array_equal(eng.perft_batch(pos, 4, mode=pg.MODE_PERSISTENT), want)

@@ -115,3 +115,25 @@ def test_serving_chunks_by_block_sums(pg, oracle):
         eng.serve_stop()
     finally:
         eng.close()
+
+
+def test_serving_no_table_leaf_parent_chunks(pg):
+    """No table (so no keys and no duplicate tables): 64 MiB per job, and the
+    ply-4 levels of depth-6 requests (about 1.5e6 nodes x 80 B each) do not
+    fit, so the ply-3 frames chunk right above the leaf pass; checked against
+    the oracle's perft 6 of the first C5 positions"""
+    with open(os.path.join(GOLDEN, "workloads.json")) as f:
+        want = json.load(f)["c5"]["perft_first8_d6"][:4]
+    from perft_gpu_amd import workloads as W
+    pos = W.positions(W.C5)[:4]
+    eng = pg.Engine(backend="ks", arena_bytes=256 << 20, tt_bytes=0)
+    try:
+        eng.serve_start(max_jobs=4, job_bytes=64 << 20)
+        for i in range(4):
+            eng.submit(pos[i], 6, i)
+        got = _drain(eng, 4, timeout=300.0)
+        assert [got[i]["status"] for i in range(4)] == [0] * 4
+        assert [got[i]["nodes"] for i in range(4)] == want
+        eng.serve_stop()
+    finally:
+        eng.close()
