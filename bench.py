@@ -411,29 +411,62 @@ def run_tt_workload(pg, W, cfg, positions, depth, backend, arena, check=None, ce
         eng.close()
 
 
-def table_semantics(pg, W, backend, c3, want8, tune_file, serve_seconds=20.0):
+def _children(pg, eng, positions, ep_every_double_push):
+    """Every legal child of every position (as pf_pos48 tuples), with the en
+    passant file set the way the run that stored them sets it."""
+    kids = []
+    for r in positions:
+        pos = pg.Pos48(*[int(x) for x in r])
+        for uci, _ in eng.divide(pos, 1):
+            c = pg.apply_uci(pos, uci)
+            if ep_every_double_push:
+                frm = (ord(uci[0]) - 97) + 8 * (ord(uci[1]) - 49)
+                to = (ord(uci[2]) - 97) + 8 * (ord(uci[3]) - 49)
+                pawns = pos.pawns & 0x00FFFFFFFFFFFF00  # ranks 2-7; the other bits hold the state
+                if (pawns >> frm) & 1 and abs(to - frm) == 16:
+                    c.pawns = (c.pawns & ~0xF0) | (((frm & 7) + 1) << 4)
+            kids.append(c.as_tuple())
+    return kids
+
+
+def table_semantics(pg, W, backend, c3, want8, tune_file):
     """The table's semantics measured (the reference's switches.h:43-48 and
-    launcher.h:62-126): C3 perft 8 with exact en-passant flagging and with
-    the file set after every double push -- in-level duplicates found,
-    nodes written, time; and serving capacity on the C4 stream's first
-    `serve_seconds` with one depth-keyed table, with four per-depth regions,
-    and with always-set flagging -- table probes, hits and hit rate."""
-    out = {"c3_d8": {}, "serving": {}}
+    launcher.h:62-126):
+      c3_d8    C3 perft 8 with exact en-passant flagging and with the file set
+               after every double push: in-level duplicates found, nodes
+               written and made, time;
+      requery  256 C5 positions at perft 5 on a 4 MiB table (2^17 buckets,
+               far fewer than the entries stored), then all their children
+               at perft 4 on the same table: the second batch's probes and
+               hits, with one depth-keyed table, with four per-depth regions,
+               and with always-set flagging."""
+    out = {"c3_d8": {}, "requery": {}}
     for name, exact in (("exact_ep", True), ("ep_every_double_push", False)):
         r = run_tt_workload(pg, W, W.C3, c3, 8, backend, 128 << 30, tune_file=tune_file,
                             check=(lambda got: int(got[0]) == want8) if want8 else None,
                             setup=lambda e, x=exact: e.set_exact_ep(x))
         out["c3_d8"][name] = {k: r[k] for k in ("seconds", "dup_nodes", "frontier_nodes", "made_nodes",
                                                 "tt_probes", "tt_hits", "counts_match") if k in r}
+    roots = W.positions(W.C5)[:256]
     for name, exact, regions in (("one_table", True, 0), ("per_depth_regions_4", True, 2),
                                  ("ep_every_double_push", False, 0)):
-        r = run_serving(pg, W, serve_seconds, backend, paced=False,
-                        setup=lambda e, x=exact, k=regions: (e.set_exact_ep(x), e.set_tt_regions(k)))
-        r.pop("_counts")
-        out["serving"][name] = {k: r[k] for k in ("requests", "nodes_per_s", "tt_probes", "tt_hits", "dup_nodes",
-                                                  "span_s")}
-        out["serving"][name]["hit_rate"] = r["tt_hits"] / max(1, r["tt_probes"])
-    out["serving_sample"] = "the C4 requests arriving in the first %.0f s, all submitted at once" % serve_seconds
+        eng = pg.Engine(backend=backend, arena_bytes=8 << 30, tt_bytes=4 << 20, zobrist_seed=W.C5["seed"])
+        try:
+            eng.set_exact_ep(exact)
+            eng.set_tt_regions(regions)
+            first = eng.perft_batch(roots, 5)
+            kids = _children(pg, eng, roots, not exact)
+            eng.reset_stats()
+            second = eng.perft_batch(kids, 4)
+            st = eng.stats()
+            out["requery"][name] = {"first_nodes": int(first.sum()), "children": len(kids),
+                                    "second_nodes": int(second.sum()), "probes": st["tt_probes"],
+                                    "hits": st["tt_hits"], "hit_rate": st["tt_hits"] / max(1, st["tt_probes"]),
+                                    "made_nodes": st["made_nodes"]}
+        finally:
+            eng.close()
+    same = len(set((v["first_nodes"], v["second_nodes"]) for v in out["requery"].values())) == 1
+    out["requery_counts_agree"] = same
     return out
 
 

@@ -244,8 +244,11 @@ static inline int wg_size(int backend) { return backend == kByte ? kByteWG : kWG
 static inline int blocks_per_cu(void* f, int backend) {
     if (backend == kByte) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_bytes(backend));
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, wg_size(backend), dyn_bytes(backend)) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, wg_size(backend), dyn_bytes(backend)) != hipSuccess ||
+        per_cu < 1) {
+        (void)hipGetLastError();  // a failed query must not surface as a later call's error
         per_cu = 1;
+    }
     return per_cu;
 }
 

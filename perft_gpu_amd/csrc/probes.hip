@@ -112,7 +112,7 @@ pf_status pf::Engine::probe(int kind, u64 bytes, u64 iters, double* out) {
         if (hipMalloc(&sink, 4) != hipSuccess) return PF_ENOMEM;
         const u32 it = iters ? (u32)iters : 4096;
         int per_cu = 0;
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_valu, 256, 0);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_valu, 256, 0) != hipSuccess) (void)hipGetLastError();
         const int grid = std::max(per_cu, 1) * n_cu * 4;
         hipLaunchKernelGGL(k_valu, dim3(grid), dim3(256), 0, stream, 16u, 1u, sink);  // warm
         hipEventRecord(a, stream);
@@ -143,7 +143,7 @@ pf_status pf::Engine::probe(int kind, u64 bytes, u64 iters, double* out) {
         default: f = (const void*)k_opc64; break;
         }
         int per_cu = 0;
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 256, 0);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 256, 0) != hipSuccess) (void)hipGetLastError();
         const int grid = std::max(per_cu, 1) * n_cu * 4;
         u32 warm = 8, itv = it;
         void* a1[] = {(void*)&warm, (void*)&sink};

@@ -635,7 +635,10 @@ int step_max_bpc(int backend, int leaf) {
             : backend == kFancy ? (const void*)&k_step<kFancy, kTTWaves, kWrite>
                                 : (const void*)&k_step<kByte, kTTWaves, kWrite>;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kTWG, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kTWG, 0) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();  // a failed query must not surface as a later call's error
+        per_cu = 1;
+    }
     return per_cu;
 }
 
@@ -928,7 +931,10 @@ static void* pick(int backend, int mode) { return mode == kLeaf ? pick_mode<kLea
 
 static int tt_bpc(void* f) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kTWG, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kTWG, 0) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();  // a failed query must not surface as a later call's error
+        per_cu = 1;
+    }
     return per_cu;
 }
 int expand_tt_max_bpc(int backend) { return tt_bpc(pick(backend, kWrite)); }
