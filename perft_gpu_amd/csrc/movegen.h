@@ -431,8 +431,18 @@ PF_HD void generate(const Board& b, const SL& sl, const PosInfo& pi, Emit& emit)
 // rook-move and rook-capture updates on every reachable position.  The child
 // gets an en-passant file only if an enemy pawn stands next to the
 // double-pushed pawn (EXACT_EN_PASSENT_FLAGGING, :3528-3547).
+// The flag is decoded from its four bits, each widened to an all-ones or
+// all-zero mask, so that no flag value costs a compare and selects:
+//   bit 3: promotion, to N (bits 1-0 = 00), B (01), R (10) or Q (11);
+//   bit 2: en passant (5 is the only flag with it);
+//   bit 1 without bit 3: castling, short (2) or long (3);
+//   bit 0 alone: double push (1).
+PF_HD u32 flag_mask(u32 m, int k) { return (u32)((int)(m << (31 - k)) >> 31); }
+PF_HD u64 widen(u32 x) { return ((u64)x << 32) | x; }
+
 PF_HD Board make_move(const Board& b, u32 m) {
-    const int from = m & 63, to = (m >> 6) & 63, fl = (m >> 12) & 15;
+    const int from = m & 63, to = (m >> 6) & 63;
+    const u32 f0 = flag_mask(m, 12), f1 = flag_mask(m, 13), f2 = flag_mask(m, 14), f3 = flag_mask(m, 15);
     const u64 src = bit(from), dst = bit(to), mv = src | dst;
     u64 P = and_andn(b.pawns, kRanks2to7, dst);
     u64 N = b.knights & ~dst, B = b.bishops & ~dst, R = b.rooks & ~dst, Kg = b.kings;
@@ -442,24 +452,26 @@ PF_HD Board make_move(const Board& b, u32 m) {
     B = xor_and(B, mv, 0 - ((B >> from) & 1));
     R = xor_and(R, mv, 0 - ((R >> from) & 1));
     Kg = xor_and(Kg, mv, 0 - ((Kg >> from) & 1));
-    if (fl & 8) {
-        P &= ~dst;
-        N |= (fl == kFlPromoN) ? dst : 0;
-        B |= (fl == kFlPromoB || fl == kFlPromoQ) ? dst : 0;
-        R |= (fl == kFlPromoR || fl == kFlPromoQ) ? dst : 0;
-    }
-    if (fl == kFlEP) P &= ~(dst >> 8);
-    if (fl == kFlOO) { R ^= 0xA0ULL; us ^= 0xA0ULL; }
-    if (fl == kFlOOO) { R ^= 0x09ULL; us ^= 0x09ULL; }
+    // promotion: the pawn leaves, the new piece appears on dst
+    P = bitop3<0x70>(P, dst, widen(f3));                                   // P & ~(dst & f3)
+    N = or_and(N, dst, widen(bitop3_32<0x10>(f3, f1, f0)));               // f3 & ~f1 & ~f0
+    B = or_and(B, dst, widen(f3 & f0));
+    R = or_and(R, dst, widen(f3 & f1));
+    // en passant: the captured pawn stands behind dst
+    P = bitop3<0x70>(P, shr<8>(dst), widen(f2));
+    // castling: the rook's two squares (h1, f1 or a1, d1) toggle in R and us
+    const u32 rook = bitop3_32<0x40>(bitop3_32<0xCA>(f0, 0x09u, 0xA0u), f1, f3);  // (f0 ? 0x09 : 0xA0) & f1 & ~f3
+    R ^= rook;
+    us ^= rook;
     int rights = (int)(b.pawns & 15);
     const int clr = (int)(((mv >> 4) & 1) * 3) | (int)((mv >> 7) & 1) | (int)((mv & 1) << 1) |
                     (int)(((mv >> 60) & 1) * 12) | (int)(((mv >> 63) & 1) << 2) | (int)(((mv >> 56) & 1) << 3);
     rights &= ~clr;
-    int ep = 0;
-    if (fl == kFlDouble) {
-        const u64 adj = ((dst << 1) & kNotA) | ((dst >> 1) & kNotH);
-        if (adj & P & ~us) ep = (from & 7) + 1;
-    }
+    // double push: an en-passant file only if an enemy pawn stands beside the
+    // pawn on its fourth rank (the low half of the board)
+    const u32 d4 = (u32)dst, side = bitop3_32<0xF8>((d4 >> 1) & 0x7F7F7F7Fu, d4 << 1, 0xFEFEFEFEu);
+    const u32 dbl = bitop3_32<0x10>(f0, f1, f2 | f3);                     // f0 & ~f1 & ~(f2 | f3)
+    const int ep = (bitop3_32<0x40>(side, (u32)P, (u32)us) & dbl) ? (from & 7) + 1 : 0;
     const u64 occ = or3(or3(P, N, B), R, Kg);
     Board c;
     c.us = flip(occ & ~us);

@@ -9,9 +9,21 @@
 // the previous launch finished writing.
 //
 // The level kernel is built in kStreamVariants forms -- two tile shapes times
-// three waves-per-SIMD targets -- for each mode, and the autotuner picks the
-// leaf and the write form per backend (the reference fixes its launch shape
-// at compile time, perft_bb.h:327-329, switches.h:14-21).
+// three waves-per-SIMD targets -- for the leaf mode and in the first five of
+// them for the write mode, and the autotuner picks the leaf and the write
+// form per backend (the reference fixes its launch shape at compile time,
+// perft_bb.h:327-329, switches.h:14-21).
+//
+// The write kernel's 256 x 4096 form built for five waves per SIMD (96
+// VGPRs, over 100 B of spills per lane) is not built: compiled from the
+// current make-move and count code with the current compiler, it gave counts
+// that differ from the oracle's on a perft 4 sample, and differently from run
+// to run.  Its other forms, the leaf kernel's five-wave forms, and the same
+// form built with only one of the two latest changes to that code (the
+// flag decoding of make_move, the branch-free diagonal masks) all match.  No
+// cause in the source was found: the LDS tile is fenced by barriers and every
+// spill slot lies inside the declared private segment.  The form was never a
+// tuned choice: the write level spills more than it gains at that target.
 #include <hip/hip_runtime.h>
 
 #include "level_dev.h"
@@ -114,7 +126,9 @@ static void* level_fn(int v) {
         case 2: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 0, 2>);
         case 3: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 1, 0>);
         case 4: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 1, 1>);
-        default: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 1, 2>);
+        default:
+            if constexpr (MODE == kWrite) return reinterpret_cast<void*>(&k_stream_level<B, MODE, 1, 1>);
+            else return reinterpret_cast<void*>(&k_stream_level<B, MODE, 1, 2>);
         }
     }
 }

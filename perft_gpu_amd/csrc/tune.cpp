@@ -31,7 +31,7 @@ void Engine::apply_tuning() {
         Tune& t = tune[b];
         if (t.p_variant < 0 || t.p_variant >= kPersistVariants || b == kByte) t.p_variant = 0;
         if (t.s_leaf < 0 || t.s_leaf >= kStreamVariants) t.s_leaf = kStreamLeafDefault;
-        if (t.s_write < 0 || t.s_write >= kStreamVariants) t.s_write = kStreamWriteDefault;
+        if (t.s_write < 0 || t.s_write >= kStreamWriteVariants) t.s_write = kStreamWriteDefault;
         if (t.tt_chunk_pct < 10 || t.tt_chunk_pct > 90) t.tt_chunk_pct = 50;
         const int pmax = persistent_max_bpc(b, t.p_variant);
         const int pb = (t.p_bpc > 0 && t.p_bpc < pmax) ? t.p_bpc : pmax;
@@ -189,8 +189,8 @@ pf_status Engine::autotune(const pf_pos48* pos, int n, int d, int what) {
         apply_tuning();
         if (s == PF_OK) {  // the stream-ordered form: its leaf kernel's forms, then its write kernel's
             double stream_ms = 1e30;
-            const int nsv = (b == kByte) ? 1 : kStreamVariants;
             for (int pass = 0; pass < 2 && s == PF_OK; pass++) {
+                const int nsv = (b == kByte) ? 1 : pass == 0 ? kStreamVariants : kStreamWriteVariants;
                 for (int v = 0; v < nsv; v++) {
                     if (pass == 0) tune[b].s_leaf = v; else tune[b].s_write = v;
                     apply_tuning();
