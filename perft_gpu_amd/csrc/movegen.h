@@ -96,15 +96,20 @@ PF_HD void analyse(const Board& b, const SL& sl, PosInfo& pi) {
     pi.att = leaper_threats(b, pi) | sl.bishop_multi(b.bishops & pi.them, occX) | sl.rook_multi(b.rooks & pi.them, occX);
 }
 
-// Knight moves of every knight in `n` onto `tgt`: the eight jump directions
-// are translations, so two knights never share a target within a direction.
-// Each jump is one shift and one three-input and with its wrap mask.
-PF_HD int knight_count(u64 n, u64 tgt) {
+// Knight moves of every knight in `n` onto `tgt`, added to `acc`: the eight
+// jump directions are translations, so two knights never share a target
+// within a direction.  Each jump is one shift and one three-input and with
+// its wrap mask.
+PF_HD u32 knight_count(u64 n, u64 tgt, u32 acc) {
     constexpr u64 kNotAB = 0xfcfcfcfcfcfcfcfcULL, kNotGH = 0x3f3f3f3f3f3f3f3fULL;
-    return popc(and3(shl<17>(n), kNotA, tgt)) + popc(and3(shl<15>(n), kNotH, tgt)) +
-           popc(and3(shl<10>(n), kNotAB, tgt)) + popc(and3(shl<6>(n), kNotGH, tgt)) +
-           popc(and3(shr<15>(n), kNotA, tgt)) + popc(and3(shr<17>(n), kNotH, tgt)) +
-           popc(and3(shr<6>(n), kNotAB, tgt)) + popc(and3(shr<10>(n), kNotGH, tgt));
+    acc = popc_add(and3(shl<17>(n), kNotA, tgt), acc);
+    acc = popc_add(and3(shl<15>(n), kNotH, tgt), acc);
+    acc = popc_add(and3(shl<10>(n), kNotAB, tgt), acc);
+    acc = popc_add(and3(shl<6>(n), kNotGH, tgt), acc);
+    acc = popc_add(and3(shr<15>(n), kNotA, tgt), acc);
+    acc = popc_add(and3(shr<17>(n), kNotH, tgt), acc);
+    acc = popc_add(and3(shr<6>(n), kNotAB, tgt), acc);
+    return popc_add(and3(shr<10>(n), kNotGH, tgt), acc);
 }
 
 // Slider moves onto `tgt` for bishops-like set B and rook-like set R, with
@@ -164,9 +169,9 @@ PF_HD CountMasks count_masks(const PosInfo& pi) {
     return m;
 }
 
-// King moves and castling, given the threat map `att`.
-PF_HD int count_king(const Board& b, const PosInfo& pi, const CountMasks& cm, u64 att) {
-    int n = popc(andn2(king_fill(pi.K), pi.us, att));
+// King moves and castling, given the threat map `att`, added to `n`.
+PF_HD u32 count_king(const Board& b, const PosInfo& pi, const CountMasks& cm, u64 att, u32 n) {
+    n = popc_add(andn2(king_fill(pi.K), pi.us, att), n);
     // castling (MoveGeneratorBitboard.h:3180-3198), never out of check
     const int r = st_rights(b);
     n += (!cm.check && (r & 1) && !(pi.occ & kF1G1) && !(att & kF1G1)) ? 1 : 0;
@@ -174,8 +179,8 @@ PF_HD int count_king(const Board& b, const PosInfo& pi, const CountMasks& cm, u6
     return n;
 }
 
-// Pawn, en-passant and knight moves.
-PF_HD int count_pawns_knights(const Board& b, const PosInfo& pi, const CountMasks& cm);
+// Pawn, en-passant and knight moves, added to `n`.
+PF_HD u32 count_pawns_knights(const Board& b, const PosInfo& pi, const CountMasks& cm, u32 n);
 
 // Number of legal moves of the Kogge-Stone backend: the threat map's slider
 // rays and our sliders' moves are filled direction by direction over one
@@ -195,12 +200,12 @@ PF_HD u32 count_moves_ks(const Board& b) {
     const int ksq = pi.ksq;
     const u64 rv = either_and(free, file_of(ksq), oR), rh = either_and(free, rank_of(ksq), oR);
     const u64 bd = either_and(free, diag_of(ksq), oB), ba = either_and(free, anti_of(ksq), oB);
-    int n = 0;
+    u32 n = 0;
     // each direction: the enemy ray (threat map) and our moves' count
-#define PF_KS_DIR(D, C, E, O, RAY)                                          \
-    {                                                                        \
-        RAY = step<D>(ray_reach<D>(E, C));                                   \
-        n += popc(and3(step<D>(ray_reach<D>(O, C)), tgt, step_mask<D>()));   \
+#define PF_KS_DIR(D, C, E, O, RAY)                                              \
+    {                                                                            \
+        RAY = step<D>(ray_reach<D>(E, C));                                       \
+        n = popc_add(and3(step<D>(ray_reach<D>(O, C)), tgt, step_mask<D>()), n); \
     }
     u64 rN, rS, rE, rW, rNE, rSW, rNW, rSE;
     {
@@ -236,7 +241,7 @@ PF_HD u32 count_moves_ks(const Board& b) {
     // the rays that still need the kNotA / kNotH wrap mask, then the map
     const u64 attA = or3(rE, rNE, rSE), attH = or3(rW, rSW, rNW);
     att = or_and(or_and(or3(att, rN, rS), attA, kNotA), attH, kNotH);
-    return (u32)(n + count_king(b, pi, cm, att) + count_pawns_knights(b, pi, cm));
+    return count_pawns_knights(b, pi, cm, count_king(b, pi, cm, att, n));
 }
 
 template <class SL>
@@ -248,20 +253,19 @@ PF_HD u32 count_moves(const Board& b, const SL& sl) {
         analyse(b, sl, pi);
         const CountMasks cm = count_masks(pi);
         const u64 frozen = pi.pinned & ~cm.pinMov;
-        return (u32)(count_king(b, pi, cm, pi.att) + count_pawns_knights(b, pi, cm) +
-                     slider_count(sl, b.bishops & pi.us & ~frozen, b.rooks & pi.us & ~frozen, pi.occ, cm.tgt,
-                                  cm.pinMov, pi.ksq));
+        const u32 ns = (u32)slider_count(sl, b.bishops & pi.us & ~frozen, b.rooks & pi.us & ~frozen, pi.occ, cm.tgt,
+                                         cm.pinMov, pi.ksq);
+        return count_pawns_knights(b, pi, cm, count_king(b, pi, cm, pi.att, ns));
     }
 }
 
-PF_HD int count_pawns_knights(const Board& b, const PosInfo& pi, const CountMasks& cm) {
+PF_HD u32 count_pawns_knights(const Board& b, const PosInfo& pi, const CountMasks& cm, u32 n) {
     const u64 occ = pi.occ, us = pi.us, them = pi.them, empty = ~occ;
     const u64 P = pi.pawns & us;
     const u64 Pf = and_andn(pi.pawns, us, pi.pinned);
     const int ksq = pi.ksq;
     const bool check = cm.check;
     const u64 safe = cm.safe, pinMov = cm.pinMov;
-    int n = 0;
 
     // pawns (MoveGeneratorBitboard.h:3109-3178; in check :2892-2940)
     {
@@ -272,13 +276,21 @@ PF_HD int count_pawns_knights(const Board& b, const PosInfo& pi, const CountMask
         const u64 pm = P & pinMov;
         const u64 push = or_and(Pf, pm, file_of(ksq));
         const u64 capR = or_and(Pf, pm, diag_of(ksq)), capL = or_and(Pf, pm, anti_of(ksq));
+        // single and double pushes never share a target (a double push
+        // needs the square in front of it empty, a single push to the same
+        // square a pawn there), so they are counted in one popcount
         const u64 p1 = (push << 8) & empty;
-        n += popc(p1 & safe) + 3 * popc(p1 & safe & kRank8);
         const u64 dmask = check ? ~safe : ~0ULL;
-        n += popc((((p1 & dmask) & kRank3) << 8) & empty & safe);
+        const u64 p2 = (((p1 & dmask) & kRank3) << 8) & empty;
+        n = popc_add(either_and(p1, p2, safe), n);
         const u64 ts = them & safe;
         const u64 cl = and3(shl<7>(capL), kNotH, ts), cr = and3(shl<9>(capR), kNotA, ts);
-        n += popc(cl) + popc(cr) + 3 * (popc(cl & kRank8) + popc(cr & kRank8));
+        n = popc_add(cr, popc_add(cl, n));
+        // promotions count four: three more for each move onto the last rank
+        // (the board's upper half)
+        const u32 promo = popc_add32((u32)((p1 & safe) >> 32) & 0xFF000000u,
+                                     popc_add32((u32)(cl >> 32) & 0xFF000000u, popc_add32((u32)(cr >> 32) & 0xFF000000u, 0u)));
+        n += 3 * promo;
     }
     // en passant (MoveGeneratorBitboard.h:3058-3107; in check :2873-2890)
     const int ep = st_ep(b);
@@ -287,7 +299,7 @@ PF_HD int count_pawns_knights(const Board& b, const PosInfo& pi, const CountMask
         const u64 src = (((C << 1) & kNotA) | ((C >> 1) & kNotH)) & P;
         if (check) {
             // only when the pawn that just moved is the single checker
-            if (C == pi.checkers) n += popc(src & ~pi.pinned);
+            if (C == pi.checkers) n = popc_add(src & ~pi.pinned, n);
         } else {
             const u64 eR = b.rooks & them;
             u64 ok = src & ~pi.pinned;
@@ -300,7 +312,7 @@ PF_HD int count_pawns_knights(const Board& b, const PosInfo& pi, const CountMask
                     n += ((att_e(eR, pro) | att_w(eR, pro)) & pi.K) ? 0 : 1;
                 }
             } else {
-                n += popc(ok);
+                n = popc_add(ok, n);
             }
             u64 pin = src & pi.pinned;  // a pinned capturer only along its pin line
             while (pin) {
@@ -315,8 +327,7 @@ PF_HD int count_pawns_knights(const Board& b, const PosInfo& pi, const CountMask
         }
     }
     // knights (pinned ones never move)
-    n += knight_count(and_andn(b.knights, us, pi.pinned), cm.tgt);
-    return n;
+    return knight_count(and_andn(b.knights, us, pi.pinned), cm.tgt, n);
 }
 
 // Legal move generation.  `emit(move)` is called once per legal move; the
