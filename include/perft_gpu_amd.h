@@ -53,7 +53,7 @@ typedef enum pf_backend {
     PF_BACKEND_KS = 0,     /* Kogge-Stone sliders (USE_SLIDING_LUT 0) */
     PF_BACKEND_PLAIN = 1,  /* plain magics, 2.25 MiB (USE_FANCY_MAGICS 0) */
     PF_BACKEND_FANCY = 2,  /* fixed-shift fancy magics, 760 KiB (reference default) */
-    PF_BACKEND_BYTE = 3    /* byte-lookup fancy magics, 146 KiB, staged in LDS */
+    PF_BACKEND_BYTE = 3    /* byte-lookup fancy magics, 146 KiB */
 } pf_backend;
 
 typedef enum pf_mode {

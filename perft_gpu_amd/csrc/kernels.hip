@@ -31,19 +31,17 @@ namespace pf {
 // levels, and for small ones (few parents per workgroup, latency-bound) the
 // 256 x 8192 tile without a waves-per-SIMD target, which does not spill.
 template <int B, int MODE, int V>
-__global__ __launch_bounds__(WgOf<B>::v)
+__global__ __launch_bounds__(kWG)
 __attribute__((amdgpu_waves_per_eu(V == LevelV<B>::v ? LevelWaves<B, MODE>::v : 1)))
 void k_expand(const SliderTables* tabs, LevelArgs a) {
-    extern __shared__ __align__(16) unsigned char dyn_lds[];
-    __shared__ TileLds<PCfg<B, V>::TP, PCfg<B, V>::CAP, WgOf<B>::v> s;
-    const auto sl = stage_backend<B>(tabs, dyn_lds);
+    __shared__ TileLds<PCfg<B, V>::TP, PCfg<B, V>::CAP, kWG> s;
+    const auto sl = SlOf<B>::make(tabs);
     expand_level(sl, a, MODE, s);
 }
 
 template <int B>
 __global__ __launch_bounds__(kWG) void k_init_roots(const SliderTables* t, Board* roots, u32 n,
                                                     unsigned long long* sums) {
-    // roots are few: the byte backend reads its tables from global memory here
     const typename SlOf<B>::T sl = SlOf<B>::make(t);
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -93,12 +91,11 @@ __global__ __launch_bounds__(kWG) void k_range_count(const Board* b, u32 n, unsi
 // All branch decisions between barriers read the same device words, so every
 // workgroup takes the same path and reaches every barrier.
 template <int B, int V>
-__global__ __launch_bounds__(WgOf<B>::v) void k_persistent(const SliderTables* tabs, PersistArgs p) {
+__global__ __launch_bounds__(kWG) void k_persistent(const SliderTables* tabs, PersistArgs p) {
     namespace cg = cooperative_groups;
     cg::grid_group grid = cg::this_grid();
-    extern __shared__ __align__(16) unsigned char dyn_lds[];
-    __shared__ TileLds<PCfg<B, V>::TP, PCfg<B, V>::CAP, WgOf<B>::v> s;
-    const auto sl = stage_backend<B>(tabs, dyn_lds);
+    __shared__ TileLds<PCfg<B, V>::TP, PCfg<B, V>::CAP, kWG> s;
+    const auto sl = SlOf<B>::make(tabs);
     PersistCtl* ctl = p.ctl;
 
     // level 0: root move counts
@@ -159,7 +156,7 @@ __global__ __launch_bounds__(WgOf<B>::v) void k_persistent(const SliderTables* t
 // ---- launch plumbing --------------------------------------------------------
 template <int B, int MODE>
 static void* expand_fn(int small) {
-    if (B == kByte || small) return reinterpret_cast<void*>(&k_expand<B, MODE, 0>);
+    if (small) return reinterpret_cast<void*>(&k_expand<B, MODE, 0>);
     return reinterpret_cast<void*>(&k_expand<B, MODE, LevelV<B>::v>);
 }
 
@@ -186,15 +183,15 @@ static void* pick_persistent(int backend, int variant) {
     case kKS: return persistent_fn<kKS>(variant);
     case kPlain: return persistent_fn<kPlain>(variant);
     case kFancy: return persistent_fn<kFancy>(variant);
-    default: return reinterpret_cast<void*>(&k_persistent<kByte, 0>);
+    default: return persistent_fn<kByte>(variant);
     }
 }
 
 
 int expand_grid(int backend, int mode, int small, int n_cu) {
-    return blocks_per_cu(pick_expand(backend, mode, small), backend) * n_cu;
+    return blocks_per_cu(pick_expand(backend, mode, small)) * n_cu;
 }
-int persistent_max_bpc(int backend, int variant) { return blocks_per_cu(pick_persistent(backend, variant), backend); }
+int persistent_max_bpc(int backend, int variant) { return blocks_per_cu(pick_persistent(backend, variant)); }
 
 hipError_t launch_init_roots(int backend, const DevTables& dt, Board* roots, u32 n,
                              unsigned long long* sums, hipStream_t s) {
@@ -216,7 +213,7 @@ hipError_t launch_expand(int backend, int mode, int small, const DevTables& dt, 
     LevelArgs args = a;
     const SliderTables* t = dt.t;
     void* kargs[] = {(void*)&t, (void*)&args};
-    return hipLaunchKernel(f, dim3(grid), dim3(wg_size(backend)), kargs, dyn_bytes(backend), s);
+    return hipLaunchKernel(f, dim3(grid), dim3(kWG), kargs, 0, s);
 }
 
 hipError_t launch_persistent(int backend, int variant, const DevTables& dt, const PersistArgs& a, int grid,
@@ -225,7 +222,7 @@ hipError_t launch_persistent(int backend, int variant, const DevTables& dt, cons
     PersistArgs args = a;
     const SliderTables* t = dt.t;
     void* kargs[] = {(void*)&t, (void*)&args};
-    return hipLaunchCooperativeKernel(f, dim3(grid), dim3(wg_size(backend)), kargs, (unsigned)dyn_bytes(backend), s);
+    return hipLaunchCooperativeKernel(f, dim3(grid), dim3(kWG), kargs, 0, s);
 }
 
 hipError_t launch_range_count(const Board* b, u32 n, unsigned long long* out, hipStream_t s) {

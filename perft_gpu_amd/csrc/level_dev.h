@@ -9,19 +9,14 @@
 
 namespace pf {
 
+// Every level kernel runs 256-thread workgroups, several per CU.  The
+// magic backends, byte lookup included, read their tables through the
+// caches: the byte backend's 146 KiB of tables once sat staged in each
+// workgroup's LDS, which left room for one (768-thread) workgroup per CU, and
+// its dependent pairs of LDS reads then waited for want of other waves.  On
+// the 256 x perft 5 batch it ran 6.3e11 leaves/s that way, and 1.17e12 with
+// five cache-reading workgroups per CU.
 constexpr int kWG = 256;
-
-// The byte-lookup backend fills nearly all of a CU's LDS with its tables, so a
-// CU holds a single one of its workgroups.  That workgroup is larger (12
-// waves, three per SIMD) so that other waves hide each wave's table-lookup
-// latency; the other backends run several 256-thread workgroups per CU.
-// Measured on C2 (256 positions, perft 5): 256 threads 3.7e11 leaves/s,
-// 512 5.4e11, 768 6.2e11, 1024 5.9e11 (all with the 64 x 1920 tile below).
-constexpr int kByteWG = 768;
-template <int B>
-struct WgOf {
-    static constexpr int v = (B == kByte) ? kByteWG : kWG;
-};
 
 __device__ __forceinline__ u32 wave_incl_scan(u32 v, int lane) {
 #pragma unroll
@@ -36,39 +31,6 @@ __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
-}
-
-// Byte backend: the 148 KB lookup blob and the 4 KB of magic entries are copied
-// into LDS once per workgroup; the workgroup is persistent, so the copy is
-// amortised over every tile it processes.
-constexpr size_t kByteBlob = offsetof(SliderTables, byte_b) + sizeof(u64) * 1428 - offsetof(SliderTables, byte_tab);
-constexpr size_t kByteEntries = sizeof(MagicEntry) * 128;
-constexpr size_t kByteLds = kByteBlob + kByteEntries;
-static_assert(kByteBlob % 16 == 0 && offsetof(SliderTables, byte_tab) % 16 == 0, "16-byte staging");
-
-template <int B>
-__device__ __forceinline__ typename SlOf<B>::T stage_backend(const SliderTables* t, unsigned char* dyn) {
-    if constexpr (B == kByte) {
-        const uint4* se = reinterpret_cast<const uint4*>(t->fancy_b);
-        uint4* de = reinterpret_cast<uint4*>(dyn);
-        for (u32 i = threadIdx.x; i < kByteEntries / 16; i += blockDim.x) de[i] = se[i];
-        const uint4* sb = reinterpret_cast<const uint4*>(t->byte_tab);
-        uint4* db = reinterpret_cast<uint4*>(dyn + kByteEntries);
-        for (u32 i = threadIdx.x; i < kByteBlob / 16; i += blockDim.x) db[i] = sb[i];
-        __syncthreads();
-        const MagicEntry* e = reinterpret_cast<const MagicEntry*>(dyn);
-        const unsigned char* base = dyn + kByteEntries;
-        ByteBackend bb;
-        bb.eb = e;
-        bb.er = e + 64;
-        bb.bt = base;
-        bb.ur = reinterpret_cast<const u64*>(base + (offsetof(SliderTables, byte_r) - offsetof(SliderTables, byte_tab)));
-        bb.ub = reinterpret_cast<const u64*>(base + (offsetof(SliderTables, byte_b) - offsetof(SliderTables, byte_tab)));
-        return bb;
-    } else {
-        (void)dyn;
-        return SlOf<B>::make(t);
-    }
 }
 
 // Emitter used by generate(): stores the moves of one parent whose index in
@@ -185,48 +147,25 @@ __device__ __forceinline__ void expand_level(const SL& sl, const LevelArgs& a, c
     }
 }
 
-template <int B>
-struct Cfg {
-    static constexpr int TP = 256;
-    static constexpr int CAP = 8192;
-    static constexpr size_t dyn = 0;
-};
-// Byte backend tile: 64 parents and a 1920-entry move window, the largest
-// window that fits beside the staged tables, so that a tile's moves (about 30
-// per parent) mostly fit one window and each parent is generated once.
-template <>
-struct Cfg<kByte> {
-    static constexpr int TP = 64;
-    static constexpr int CAP = 1920;
-    static constexpr size_t dyn = kByteLds;
-};
-static_assert(sizeof(TileLds<Cfg<kByte>::TP, Cfg<kByte>::CAP, kByteWG>) + kByteLds + 128 <= 160 * 1024,
-              "byte backend: tables plus tile state must fit one CU's LDS");
-
 // Tile shapes of the persistent kernel that the tuner chooses between:
 // variant 0 = 256 parents x 8192-entry move window, 1 = 128 x 4096,
-// 2 = 256 x 4096.  The byte-lookup backend keeps its LDS-staged shape.
+// 2 = 256 x 4096.
 template <int B, int V>
 struct PCfg {
     static constexpr int TP = (V == 1) ? 128 : 256;
     static constexpr int CAP = (V == 0) ? 8192 : 4096;
 };
-template <int V>
-struct PCfg<kByte, V> {
-    static constexpr int TP = Cfg<kByte>::TP;
-    static constexpr int CAP = Cfg<kByte>::CAP;
-};
 
 // Tile shape of the per-level kernels (k_expand, k_stream_level): 256 x 4096
 // (variant 2), whose smaller LDS footprint lets more workgroups share a CU
-// than the 256 x 8192 shape.  The byte backend keeps its own shape.
+// than the 256 x 8192 shape.
 template <int B>
 struct LevelV {
-    static constexpr int v = (B == kByte) ? 0 : 2;
+    static constexpr int v = 2;
 };
 // Waves per SIMD the per-level kernels are built for (1 = no constraint).
 // The leaf kernel is built for five, so five workgroups share a CU; the magic
-// backends' fits 96 VGPRs as it is, Kogge-Stone's spills 28 B per lane
+// backends' fits 96 VGPRs as it is, Kogge-Stone's spills a few bytes per lane
 // (outside its make+count loop).  C2 leaves/s of the stream-ordered form,
 // 256 x 8192 at three per CU -> 256 x 4096 at four -> at five: plain 1.05e12
 // -> 1.18e12 -> 1.30e12, fancy 1.08e12 -> 1.20e12 -> 1.31e12 (their table
@@ -234,18 +173,12 @@ struct LevelV {
 // 1.500e12.  The write kernels would spill more and stay at four.
 template <int B, int MODE>
 struct LevelWaves {
-    static constexpr int v = (B != kByte && MODE == kLeaf) ? 5 : 1;
+    static constexpr int v = (MODE == kLeaf) ? 5 : 1;
 };
-// Occupancy of one of these kernels (the byte backend's dynamic LDS is
-// enabled first).
-static inline size_t dyn_bytes(int backend) { return backend == kByte ? Cfg<kByte>::dyn : 0; }
-static inline int wg_size(int backend) { return backend == kByte ? kByteWG : kWG; }
-
-static inline int blocks_per_cu(void* f, int backend) {
-    if (backend == kByte) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_bytes(backend));
+// Workgroups of one of these kernels that fit a CU.
+static inline int blocks_per_cu(void* f) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, wg_size(backend), dyn_bytes(backend)) != hipSuccess ||
-        per_cu < 1) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kWG, 0) != hipSuccess || per_cu < 1) {
         (void)hipGetLastError();  // a failed query must not surface as a later call's error
         per_cu = 1;
     }

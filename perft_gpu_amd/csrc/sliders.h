@@ -40,7 +40,7 @@ struct SliderTables {
     u64 plain_btab[64 * 512];
     u64 plain_rtab[64 * 4096];
     u64 fancy_tab[97264];
-    // Byte-lookup blob (contiguous: staged into LDS as one piece)
+    // Byte-lookup blob: per-square byte indices, then the attack sets
     u8 byte_tab[97264];
     u8 byte_pad[16];  // keeps the u64 lists 16-byte aligned
     u64 byte_r[4900];

@@ -43,9 +43,8 @@ struct StreamWaves {
 };
 
 template <int B>
-__global__ __launch_bounds__(WgOf<B>::v) void k_stream_roots(const SliderTables* tabs, PersistArgs p) {
-    extern __shared__ __align__(16) unsigned char dyn_lds[];
-    const auto sl = stage_backend<B>(tabs, dyn_lds);
+__global__ __launch_bounds__(kWG) void k_stream_roots(const SliderTables* tabs, PersistArgs p) {
+    const auto sl = SlOf<B>::make(tabs);
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long c = 0;
     if (i < p.n_roots) {
@@ -59,18 +58,17 @@ __global__ __launch_bounds__(WgOf<B>::v) void k_stream_roots(const SliderTables*
 }
 
 template <int B, int MODE, int SH, int WV>
-__global__ __launch_bounds__(WgOf<B>::v) __attribute__((amdgpu_waves_per_eu(StreamWaves<WV>::v)))
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(StreamWaves<WV>::v)))
 void k_stream_level(const SliderTables* tabs, PersistArgs p, int L) {
     using Shape = PCfg<B, StreamShape<SH>::v>;
-    extern __shared__ __align__(16) unsigned char dyn_lds[];
-    __shared__ TileLds<Shape::TP, Shape::CAP, WgOf<B>::v> s;
+    __shared__ TileLds<Shape::TP, Shape::CAP, kWG> s;
     PersistCtl* ctl = p.ctl;
     // an earlier level did not fit: nothing below it is counted (one read per
     // workgroup, so that the whole workgroup leaves together)
     if (threadIdx.x == 0) s.ok = ctl->err;
     __syncthreads();
     if (s.ok) return;
-    const auto sl = stage_backend<B>(tabs, dyn_lds);
+    const auto sl = SlOf<B>::make(tabs);
     const Board* cur = p.roots;
     const u32* cur_r = p.root_ids;
     u64 n = p.n_roots, off = 0;
@@ -116,20 +114,15 @@ void k_stream_level(const SliderTables* tabs, PersistArgs p, int L) {
 // ---- launch plumbing --------------------------------------------------------
 template <int B, int MODE>
 static void* level_fn(int v) {
-    if constexpr (B == kByte) {
-        (void)v;
-        return reinterpret_cast<void*>(&k_stream_level<B, MODE, 0, 0>);  // its one LDS-staged shape
-    } else {
-        switch (v) {
-        case 0: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 0, 0>);
-        case 1: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 0, 1>);
-        case 2: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 0, 2>);
-        case 3: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 1, 0>);
-        case 4: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 1, 1>);
-        default:
-            if constexpr (MODE == kWrite) return reinterpret_cast<void*>(&k_stream_level<B, MODE, 1, 1>);
-            else return reinterpret_cast<void*>(&k_stream_level<B, MODE, 1, 2>);
-        }
+    switch (v) {
+    case 0: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 0, 0>);
+    case 1: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 0, 1>);
+    case 2: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 0, 2>);
+    case 3: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 1, 0>);
+    case 4: return reinterpret_cast<void*>(&k_stream_level<B, MODE, 1, 1>);
+    default:
+        if constexpr (MODE == kWrite) return reinterpret_cast<void*>(&k_stream_level<B, MODE, 1, 1>);
+        else return reinterpret_cast<void*>(&k_stream_level<B, MODE, 1, 2>);
     }
 }
 template <int B>
@@ -154,27 +147,24 @@ static void* pick_stream_roots(int backend) {
 }
 
 int stream_grid(int backend, int mode, int variant, int n_cu) {
-    return blocks_per_cu(pick_stream_level(backend, mode, variant), backend) * n_cu;
+    return blocks_per_cu(pick_stream_level(backend, mode, variant)) * n_cu;
 }
 
 hipError_t launch_stream(int backend, const DevTables& dt, const PersistArgs& a, const int grid[2],
                          const int variant[2], hipStream_t s) {
     PersistArgs args = a;
     const SliderTables* t = dt.t;
-    const unsigned dyn = (unsigned)dyn_bytes(backend);
-    const int wg = wg_size(backend);
-    if (backend == kByte) hipFuncSetAttribute(pick_stream_roots(backend), hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
     {
         void* kargs[] = {(void*)&t, (void*)&args};
-        const hipError_t e = hipLaunchKernel(pick_stream_roots(backend), dim3((a.n_roots + wg - 1) / wg), dim3(wg),
-                                             kargs, dyn, s);
+        const hipError_t e = hipLaunchKernel(pick_stream_roots(backend), dim3((a.n_roots + kWG - 1) / kWG), dim3(kWG),
+                                             kargs, 0, s);
         if (e != hipSuccess) return e;
     }
     for (int L = 0; L <= a.depth - 2; L++) {
         const int mode = (L == a.depth - 2) ? kLeaf : kWrite;
         void* kargs[] = {(void*)&t, (void*)&args, (void*)&L};
         const hipError_t e = hipLaunchKernel(pick_stream_level(backend, mode, variant[mode]), dim3(grid[mode]),
-                                             dim3(wg), kargs, dyn, s);
+                                             dim3(kWG), kargs, 0, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

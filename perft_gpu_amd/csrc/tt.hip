@@ -426,7 +426,7 @@ template <int B, int M>
 __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(M == kLeaf ? kTTLeafWaves : kTTWaves)))
 void k_expand_tt(const SliderTables* tabs, const ZTables* zt, TTArgs a) {
     __shared__ TTLds s;
-    const auto sl = SlOfT<B>::make(tabs);
+    const auto sl = SlOf<B>::make(tabs);
     a.mode = M;
     expand_tt_level(sl, a, zt, s);
 }
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(256) void k_roots_tt(const SliderTables* tabs, Boar
                                                   const u64* k2, u32 n, int depth, const uint4* tt, u64 mask,
                                                   unsigned long long* acc, unsigned long long* total,
                                                   unsigned long long* stats) {
-    const auto sl = SlOfT<B>::make(tabs);
+    const auto sl = SlOf<B>::make(tabs);
     const u32 i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const Board b = roots[i];
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
     __shared__ TTLds s;
     __shared__ TTArgs sa;
     __shared__ u32 s_tile, s_seg;
-    const auto sl = SlOfT<B>::make(tabs);
+    const auto sl = SlOf<B>::make(tabs);
     const u32 s_lo = (M == kLeaf) ? t->nseg_write : 0u, s_hi = (M == kWrite) ? t->nseg_write : t->nseg;
     const u32 first = t->tile_prefix[s_lo], total = t->tile_prefix[s_hi];
     u32* const ctr = (M == kLeaf) ? &t->tile_ctr2 : &t->tile_ctr;
@@ -587,7 +587,7 @@ __global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 ma
 // step is a captured graph, which records no memset).
 template <int B>
 __global__ __launch_bounds__(256) void k_pre_step(const SliderTables* tabs, StepTable* t, const uint4* tt, u64 mask) {
-    const auto sl = SlOfT<B>::make(tabs);
+    const auto sl = SlOf<B>::make(tabs);
     const u32 gi = blockIdx.x * 256 + threadIdx.x;
     if (gi < t->nroot) {
         const RootSeg& r = t->root[gi];

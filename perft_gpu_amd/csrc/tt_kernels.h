@@ -8,12 +8,6 @@
 
 namespace pf {
 
-// The table kernels' slider backends.  Their tiles fill the LDS, so the
-// byte-lookup backend reads its tables (the same 146 KiB) from global memory
-// through the caches instead of staging them in LDS.
-template <int B>
-using SlOfT = SlOf<B>;
-
 // Table probes and stores only for nodes with at least this remaining depth:
 // a node of remaining depth 2 costs one count pass over its children, about
 // what a random 32-byte probe plus its share of the store costs.

@@ -29,7 +29,7 @@ void Engine::apply_tuning() {
     drop_plan_graphs();  // captured passes hold the old launch grids
     for (int b = 0; b < 4; b++) {
         Tune& t = tune[b];
-        if (t.p_variant < 0 || t.p_variant >= kPersistVariants || b == kByte) t.p_variant = 0;
+        if (t.p_variant < 0 || t.p_variant >= kPersistVariants) t.p_variant = 0;
         if (t.s_leaf < 0 || t.s_leaf >= kStreamVariants) t.s_leaf = kStreamLeafDefault;
         if (t.s_write < 0 || t.s_write >= kStreamWriteVariants) t.s_write = kStreamWriteDefault;
         if (t.tt_chunk_pct < 10 || t.tt_chunk_pct > 90) t.tt_chunk_pct = 50;
@@ -161,8 +161,7 @@ pf_status Engine::autotune(const pf_pos48* pos, int n, int d, int what) {
     Tune best = keep;
     if (what & kTuneSinglePass) {
         double best_ms = 1e30;
-        const int nv = (b == kByte) ? 1 : kPersistVariants;
-        for (int v = 0; v < nv && s == PF_OK; v++) {
+        for (int v = 0; v < kPersistVariants && s == PF_OK; v++) {
             tune[b].p_variant = v;
             tune[b].p_bpc = 0;
             apply_tuning();
@@ -190,7 +189,7 @@ pf_status Engine::autotune(const pf_pos48* pos, int n, int d, int what) {
         if (s == PF_OK) {  // the stream-ordered form: its leaf kernel's forms, then its write kernel's
             double stream_ms = 1e30;
             for (int pass = 0; pass < 2 && s == PF_OK; pass++) {
-                const int nsv = (b == kByte) ? 1 : pass == 0 ? kStreamVariants : kStreamWriteVariants;
+                const int nsv = pass == 0 ? kStreamVariants : kStreamWriteVariants;
                 for (int v = 0; v < nsv; v++) {
                     if (pass == 0) tune[b].s_leaf = v; else tune[b].s_write = v;
                     apply_tuning();
