@@ -355,6 +355,18 @@ def measure_ceilings(pg):
 FRONTIER_NODE_BYTES = 80  # board 48 + root word 4 + two keys 16 + parent 4 + accumulator 8
 
 
+def issue_view(leaves, kernel_ms, backend):
+    """VALU issue rate of a C2 step against the instruction mix's limit (Kogge-Stone only: the per-leaf
+    instruction count was measured for its kernels)."""
+    from perft_gpu_amd import workloads as W
+    if backend != "ks" or kernel_ms <= 0:
+        return None
+    rate = W.C2_VALU_LANE_INSTR_PER_LEAF * leaves / 64 / (kernel_ms * 1e-3 * 1024 * 2.4e9)
+    return {"lane_instr_per_leaf": W.C2_VALU_LANE_INSTR_PER_LEAF, "wave_instr_per_simd_cycle": rate,
+            "mix_limit": W.C2_LOOP_ISSUE_LIMIT, "frac": rate / W.C2_LOOP_ISSUE_LIMIT,
+            "source": "profiles/k_stream_level_ks_c2_d5_counters.md"}
+
+
 def tt_roofline(st, seconds, ceil):
     """Roofline of a TT run over its two resources: the table and duplicate
     traffic (HBM random access) and the move counting (integer VALU).
@@ -574,13 +586,17 @@ def main():
                      # HBM bytes per step: FETCH_SIZE x2.00 + WRITE_SIZE x1.00 over the step's kernels
                      # (rocprofv3 --pmc, calibrated on a copy of known size;
                      # profiles/k_stream_level_ks_c2_d5_counters.md)
-                     "traffic": 9.522e8,
+                     "traffic": 9.22e8,
                      "traffic_source": "profiles/k_stream_level_ks_c2_d5_counters.md (rocprofv3 --pmc "
                                        "FETCH_SIZE, WRITE_SIZE over one step, calibrated on a copy)",
                      "kernel": ("k_stream_roots + %d x k_stream_level<%s>" % (depth - 1, args.backend)
                                 if single == "stream" else "k_persistent<%s>" % args.backend),
                      "kernel_ms": kernel_ms,
-                     "ops_per_leaf": W.C2_OPS_PER_LEAF},
+                     "ops_per_leaf": W.C2_OPS_PER_LEAF,
+                     # the hardware view: VALU wave-instructions issued per SIMD-cycle over the
+                     # step (counted instructions per leaf, this run's kernel time, 1,024 SIMDs at
+                     # 2.4 GHz) against what the leaf loop's instruction mix can issue
+                     "issue": issue_view(leaves, kernel_ms, args.backend)},
     }
 
     sections = set() if args.no_extras else set(s for s in args.sections.split(",") if s)

@@ -254,7 +254,12 @@ double pf_now(void);
 
 /* Diagnostics (no reference counterpart): measure a ceiling on this device.
  * kind 0: int32 VALU lane-ops/s; 1: streaming copy, read+write bytes/s over a
- * `bytes` buffer; 2: random 16-byte read-modify-writes/s over a `bytes` table. */
+ * `bytes` buffer; 2: random 16-byte read-modify-writes/s over a `bytes` table;
+ * 3: issue rate of one VALU opcode (lane-instructions/s over 8 independent
+ * chains), `iters` selecting it: 0 v_add_u32, 1 v_xor_b32, 2 v_and_b32,
+ * 3 v_bcnt_u32_b32, 4 v_lshlrev_b32, 5 v_mul_lo_u32, 6 v_lshlrev_b64,
+ * 7 v_bitop3_b32, 8 v_and_or_b32, 9 v_or3_b32, 10 v_cndmask_b32,
+ * 11 v_perm_b32, 12 v_add3_u32, 13 v_min_u32, 14 v_ashrrev_i32, 15 v_bfe_i32. */
 pf_status pf_probe(pf_ctx *ctx, int kind, uint64_t bytes, uint64_t iters, double *out);
 
 /* Seeded random-playout positions (workload generator, no reference

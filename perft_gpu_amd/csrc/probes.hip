@@ -45,10 +45,18 @@ __global__ __launch_bounds__(256) void k_valu(u32 iters, u32 seed, u32* sink) {
                  OPC " %6, %6, %8, %9" MOD "\n\t" OPC " %7, %7, %8, %9" MOD                          \
                  : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6), "+v"(a7)       \
                  : "v"(k), "v"(k2))
+// select with a scalar-register mask: d = m ? k : d
+#define PF_OP8_S(OPC)                                                                                  \
+    asm volatile(OPC " %0, %0, %8, %9\n\t" OPC " %1, %1, %8, %9\n\t" OPC " %2, %2, %8, %9\n\t"              \
+                 OPC " %3, %3, %8, %9\n\t" OPC " %4, %4, %8, %9\n\t" OPC " %5, %5, %8, %9\n\t"              \
+                 OPC " %6, %6, %8, %9\n\t" OPC " %7, %7, %8, %9"                                             \
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6), "+v"(a7)       \
+                 : "v"(k), "s"(sm))
 template <int OP>
 __global__ __launch_bounds__(256) void k_opc(u32 iters, u32* sink) {
     u32 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, a4 = a0 + 4, a5 = a0 + 5, a6 = a0 + 6, a7 = a0 + 7;
     const u32 k = blockIdx.x | 3, k2 = blockIdx.x * 5u + 1u;
+    const unsigned long long sm = 0x5555555555555555ULL ^ blockIdx.x;
     for (u32 i = 0; i < iters; i++) {
 #pragma unroll
         for (int j = 0; j < 8; j++) {
@@ -61,12 +69,19 @@ __global__ __launch_bounds__(256) void k_opc(u32 iters, u32* sink) {
             if constexpr (OP == 7) PF_OP8_3("v_bitop3_b32", " bitop3:0xf8");
             if constexpr (OP == 8) PF_OP8_3("v_and_or_b32", "");
             if constexpr (OP == 9) PF_OP8_3("v_or3_b32", "");
+            if constexpr (OP == 10) PF_OP8_S("v_cndmask_b32_e64");
+            if constexpr (OP == 11) PF_OP8_3("v_perm_b32", "");
+            if constexpr (OP == 12) PF_OP8_3("v_add3_u32", "");
+            if constexpr (OP == 13) PF_OP8("v_min_u32");
+            if constexpr (OP == 14) PF_OP8("v_ashrrev_i32");
+            if constexpr (OP == 15) PF_OP8_3("v_bfe_i32", "");
         }
     }
     if ((a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7) == 0x12345678u) sink[0] = 1;
 }
 #undef PF_OP8
 #undef PF_OP8_3
+#undef PF_OP8_S
 
 __global__ __launch_bounds__(256) void k_opc64(u32 iters, u32* sink) {
     u64 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3;
@@ -125,7 +140,8 @@ pf_status pf::Engine::probe(int kind, u64 bytes, u64 iters, double* out) {
         hipFree(sink);
     } else if (kind == 3) {
         // iters selects the opcode: 0 add, 1 xor, 2 and, 3 bcnt, 4 lshl32, 5 mul_lo, 6 lshl64,
-        // 7 bitop3 (d | k & k2), 8 and_or, 9 or3
+        // 7 bitop3 (d | k & k2), 8 and_or, 9 or3, 10 cndmask (scalar mask), 11 perm, 12 add3, 13 min_u32,
+        // 14 ashrrev_i32, 15 bfe_i32
         u32* sink = nullptr;
         if (hipMalloc(&sink, 4) != hipSuccess) return PF_ENOMEM;
         const u32 it = 2048;
@@ -140,6 +156,12 @@ pf_status pf::Engine::probe(int kind, u64 bytes, u64 iters, double* out) {
         case 7: f = (const void*)k_opc<7>; break;
         case 8: f = (const void*)k_opc<8>; break;
         case 9: f = (const void*)k_opc<9>; break;
+        case 10: f = (const void*)k_opc<10>; break;
+        case 11: f = (const void*)k_opc<11>; break;
+        case 12: f = (const void*)k_opc<12>; break;
+        case 13: f = (const void*)k_opc<13>; break;
+        case 14: f = (const void*)k_opc<14>; break;
+        case 15: f = (const void*)k_opc<15>; break;
         default: f = (const void*)k_opc64; break;
         }
         int per_cu = 0;

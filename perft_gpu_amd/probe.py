@@ -13,7 +13,8 @@ from . import Engine
 
 KINDS = {"valu": 0, "copy": 1, "random16": 2, "opcodes": 3}
 OPCODES = ["v_add_u32", "v_xor_b32", "v_and_b32", "v_bcnt_u32_b32", "v_lshlrev_b32", "v_mul_lo_u32", "v_lshlrev_b64",
-           "v_bitop3_b32", "v_and_or_b32", "v_or3_b32"]
+           "v_bitop3_b32", "v_and_or_b32", "v_or3_b32", "v_cndmask_b32", "v_perm_b32", "v_add3_u32", "v_min_u32",
+           "v_ashrrev_i32", "v_bfe_i32"]
 
 
 def main():

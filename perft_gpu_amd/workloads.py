@@ -97,3 +97,13 @@ C2_LEAVES = 8988346871
 # and counted node, used to price the counting work of table runs.
 C2_MADE_NODES = 8001 + 249956 + 8164767 + 265225172
 C2_OPS_PER_MADE_NODE = 416777397783 / C2_MADE_NODES
+# What the hardware issues for the same step with the Kogge-Stone kernels
+# (rocprofv3 --pmc SQ_INSTS_VALU on one C2 step: 3.3140e9 wave-instructions in
+# the leaf level, 1.1059e8 in the three write levels), per leaf, as lane-
+# instructions; and the most wave-instructions per SIMD-cycle the leaf loop's
+# instruction mix can issue: 509 of its 841 VALU instructions issue every
+# ~2.4 cycles (two-input logic, bitop3, add, moves), 332 every ~4.2 (shifts,
+# popcounts, selects, compares, perms), measured per opcode with all waves
+# resident, i.e. 841 instructions per 2,616 cycles.
+C2_VALU_LANE_INSTR_PER_LEAF = (3.3140e9 + 1.1059e8) * 64 / 8988346871
+C2_LOOP_ISSUE_LIMIT = 841 / 2616

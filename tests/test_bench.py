@@ -34,6 +34,22 @@ def test_tt_roofline_takes_the_larger_phase():
     json.dumps(r)
 
 
+def test_issue_view_divides_counted_instructions_by_simd_cycles():
+    """a C2 step's issue rate: instructions per leaf x leaves / 64 lanes over
+    1,024 SIMDs x 2.4 GHz x the kernel time, against the loop mix's limit;
+    only Kogge-Stone has a measured instruction count"""
+    sys.path.insert(0, ROOT)
+    import bench
+    from perft_gpu_amd import workloads as W
+    r = bench.issue_view(W.C2_LEAVES, 5.0, "ks")
+    want = W.C2_VALU_LANE_INSTR_PER_LEAF * W.C2_LEAVES / 64 / (5e-3 * 1024 * 2.4e9)
+    assert abs(r["wave_instr_per_simd_cycle"] - want) < 1e-12
+    assert abs(r["frac"] - want / W.C2_LOOP_ISSUE_LIMIT) < 1e-12
+    assert 0.25 < W.C2_LOOP_ISSUE_LIMIT < 0.42  # between all single-rate and all double-rate issue
+    assert bench.issue_view(W.C2_LEAVES, 5.0, "fancy") is None
+    json.dumps(r)
+
+
 def test_op_model_per_made_node_matches_the_fixture():
     """the C2 tree's made nodes are its perft 1..4 sums, and the op total is
     the fixture's"""
