@@ -286,10 +286,12 @@ PF_HD u32 count_pawns_knights(const Board& b, const PosInfo& pi, const CountMask
         const u64 ts = them & safe;
         const u64 cl = and3(shl<7>(capL), kNotH, ts), cr = and3(shl<9>(capR), kNotA, ts);
         n = popc_add(cr, popc_add(cl, n));
-        // promotions count four: three more for each move onto the last rank
-        // (the board's upper half)
-        const u32 promo = popc_add32((u32)((p1 & safe) >> 32) & 0xFF000000u,
-                                     popc_add32((u32)(cl >> 32) & 0xFF000000u, popc_add32((u32)(cr >> 32) & 0xFF000000u, 0u)));
+        // promotions count four: three more for each move onto the last rank,
+        // the top byte of the board's upper half
+        constexpr u32 kR8 = 0xFF000000u;
+        u32 promo = popc_add32((u32)(cr >> 32) & kR8, 0u);
+        promo = popc_add32((u32)(cl >> 32) & kR8, promo);
+        promo = popc_add32((u32)((p1 & safe) >> 32) & kR8, promo);
         n += 3 * promo;
     }
     // en passant (MoveGeneratorBitboard.h:3058-3107; in check :2873-2890)
