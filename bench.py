@@ -50,6 +50,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import threading
 import time
 
@@ -652,6 +653,22 @@ def main():
             be[b] = {"value": leaves * reps / (time.perf_counter() - t), "counts_match": bool((c == counts).all())}
             log("backend %s: %.3e nodes/s" % (b, be[b]["value"]))
         eng.set_backend(args.backend)
+        # the byte backend's other form: its tables staged in each workgroup's LDS
+        with tempfile.TemporaryDirectory() as td:
+            tf = os.path.join(td, "byte_lds.txt")
+            with open(tf, "w") as f:
+                f.write("perft_gpu_amd tuning 1\nbackend byte stream_leaf 6 stream_write 6\n")
+            e2 = pg.Engine(backend="byte", arena_bytes=16 << 30, tune_file=tf)
+            try:
+                e2.perft_batch(pos, depth, mode=pg.MODE_STREAM)
+                t = time.perf_counter()
+                for _ in range(3):
+                    c = e2.perft_batch(pos, depth, mode=pg.MODE_STREAM)
+                be["byte_tables_in_lds"] = {"value": leaves * 3 / (time.perf_counter() - t),
+                                            "counts_match": bool((c == counts).all())}
+            finally:
+                e2.close()
+        log("backend byte, tables in LDS: %.3e nodes/s" % be["byte_tables_in_lds"]["value"])
         for name, mode in (("stream_mode_", pg.MODE_STREAM), ("persistent_mode_", pg.MODE_PERSISTENT),
                            ("levels_mode_", pg.MODE_LEVELS)):
             eng.perft_batch(pos, depth, mode=mode)

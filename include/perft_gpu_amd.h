@@ -186,8 +186,9 @@ pf_status pf_set_tt_regions(pf_ctx *ctx, int log2_regions);
  * the current backend on `sample` (n positions, `depth`) over
  *   - the table-free single passes: the cooperative kernel's tile shapes and
  *     workgroups per CU, the stream-ordered level kernels' forms (tile shape x
- *     waves per SIMD, leaf and write level separately) and which of the two
- *     passes PF_MODE_AUTO runs;
+ *     waves per SIMD, leaf and write level separately; for the byte-lookup
+ *     backend also the form with its tables staged in LDS, form 6) and which
+ *     of the two passes PF_MODE_AUTO runs;
  *   - with a table: the level driver's write- and leaf-level workgroups per CU
  *     and, if the sample makes it chunk, the share of free memory a chunk's
  *     next level may take;

@@ -83,6 +83,7 @@ hipError_t launch_persistent(int backend, int variant, const DevTables& dt, cons
 // per SIMD target; the byte backend has one form).
 constexpr int kStreamVariants = 6;
 constexpr int kStreamWriteVariants = 5;  // the write kernel has no five-wave 256 x 4096 form
+constexpr int kStreamByteLds = 6;        // byte backend only: its tables staged in LDS (both modes)
 constexpr int kStreamLeafDefault = 5;   // 256 x 4096 tile, five waves per SIMD
 constexpr int kStreamWriteDefault = 3;  // 256 x 4096 tile, no waves target
 int stream_grid(int backend, int mode, int variant, int n_cu);

@@ -9,13 +9,10 @@
 
 namespace pf {
 
-// Every level kernel runs 256-thread workgroups, several per CU.  The
+// Every level kernel runs 256-thread workgroups, several per CU, and the
 // magic backends, byte lookup included, read their tables through the
-// caches: the byte backend's 146 KiB of tables once sat staged in each
-// workgroup's LDS, which left room for one (768-thread) workgroup per CU, and
-// its dependent pairs of LDS reads then waited for want of other waves.  On
-// the 256 x perft 5 batch it ran 6.3e11 leaves/s that way, and 1.17e12 with
-// five cache-reading workgroups per CU.
+// caches.  The byte backend's stream-ordered kernel has one more form, its
+// 146 KiB of tables staged in the workgroup's LDS (below).
 constexpr int kWG = 256;
 
 __device__ __forceinline__ u32 wave_incl_scan(u32 v, int lane) {
@@ -31,6 +28,41 @@ __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
+}
+
+// The byte backend's LDS form: the 148 KB lookup blob and the 4 KB of magic
+// entries are copied into LDS once per workgroup (the workgroup is persistent,
+// so the copy is amortised over every tile it processes).  That leaves room
+// for one workgroup per CU, so it is 768 threads wide (three waves per SIMD)
+// and its tile is 64 parents x 1920 moves, the largest that fits beside the
+// tables.  On the 256 x perft 5 batch it runs 6.3e11 leaves/s against 1.15e12
+// for the cache-reading form with five 256-thread workgroups per CU: every
+// lookup is two dependent reads, and three waves per SIMD cannot hide them.
+constexpr int kByteLdsWG = 768;
+constexpr int kByteLdsTP = 64;
+constexpr int kByteLdsCap = 1920;
+constexpr size_t kByteBlob = offsetof(SliderTables, byte_b) + sizeof(u64) * 1428 - offsetof(SliderTables, byte_tab);
+constexpr size_t kByteEntries = sizeof(MagicEntry) * 128;
+constexpr size_t kByteLds = kByteBlob + kByteEntries;
+static_assert(kByteBlob % 16 == 0 && offsetof(SliderTables, byte_tab) % 16 == 0, "16-byte staging");
+
+__device__ __forceinline__ ByteBackend stage_byte_tables(const SliderTables* t, unsigned char* dyn) {
+    const uint4* se = reinterpret_cast<const uint4*>(t->fancy_b);
+    uint4* de = reinterpret_cast<uint4*>(dyn);
+    for (u32 i = threadIdx.x; i < kByteEntries / 16; i += blockDim.x) de[i] = se[i];
+    const uint4* sb = reinterpret_cast<const uint4*>(t->byte_tab);
+    uint4* db = reinterpret_cast<uint4*>(dyn + kByteEntries);
+    for (u32 i = threadIdx.x; i < kByteBlob / 16; i += blockDim.x) db[i] = sb[i];
+    __syncthreads();
+    const MagicEntry* e = reinterpret_cast<const MagicEntry*>(dyn);
+    const unsigned char* base = dyn + kByteEntries;
+    ByteBackend bb;
+    bb.eb = e;
+    bb.er = e + 64;
+    bb.bt = base;
+    bb.ur = reinterpret_cast<const u64*>(base + (offsetof(SliderTables, byte_r) - offsetof(SliderTables, byte_tab)));
+    bb.ub = reinterpret_cast<const u64*>(base + (offsetof(SliderTables, byte_b) - offsetof(SliderTables, byte_tab)));
+    return bb;
 }
 
 // Emitter used by generate(): stores the moves of one parent whose index in
@@ -56,6 +88,8 @@ struct TileLds {
     unsigned long long red[WG / 64];
     u32 tile, base, ok;
 };
+static_assert(sizeof(TileLds<kByteLdsTP, kByteLdsCap, kByteLdsWG>) + kByteLds + 128 <= 160 * 1024,
+              "byte backend's LDS form: tables plus tile state must fit one CU's LDS");
 
 // Process every tile of one level (tiles come from a.tile_ctr).
 template <class SL, int TP, int CAP, int WG>
@@ -175,10 +209,12 @@ template <int B, int MODE>
 struct LevelWaves {
     static constexpr int v = (MODE == kLeaf) ? 5 : 1;
 };
-// Workgroups of one of these kernels that fit a CU.
-static inline int blocks_per_cu(void* f) {
+// Workgroups of one of these kernels that fit a CU (a kernel with dynamic LDS
+// has it enabled first).
+static inline int blocks_per_cu(void* f, int wg = kWG, size_t dyn = 0) {
+    if (dyn) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kWG, 0) != hipSuccess || per_cu < 1) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, wg, dyn) != hipSuccess || per_cu < 1) {
         (void)hipGetLastError();  // a failed query must not surface as a later call's error
         per_cu = 1;
     }

@@ -57,18 +57,19 @@ __global__ __launch_bounds__(kWG) void k_stream_roots(const SliderTables* tabs, 
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(&p.ctl->totals[1], c);
 }
 
-template <int B, int MODE, int SH, int WV>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(StreamWaves<WV>::v)))
-void k_stream_level(const SliderTables* tabs, PersistArgs p, int L) {
-    using Shape = PCfg<B, StreamShape<SH>::v>;
-    __shared__ TileLds<Shape::TP, Shape::CAP, kWG> s;
+// One level for one workgroup of the launch.  `make_sl` builds the slider
+// backend once the workgroup knows it has work (the byte backend's LDS form
+// copies its tables into LDS there).
+template <int MODE, int TP, int CAP, int WG, class MakeSl>
+__device__ __forceinline__ void stream_level(const MakeSl& make_sl, const PersistArgs& p, int L,
+                                             TileLds<TP, CAP, WG>& s) {
     PersistCtl* ctl = p.ctl;
     // an earlier level did not fit: nothing below it is counted (one read per
     // workgroup, so that the whole workgroup leaves together)
     if (threadIdx.x == 0) s.ok = ctl->err;
     __syncthreads();
     if (s.ok) return;
-    const auto sl = SlOf<B>::make(tabs);
+    const auto sl = make_sl();
     const Board* cur = p.roots;
     const u32* cur_r = p.root_ids;
     u64 n = p.n_roots, off = 0;
@@ -86,7 +87,7 @@ void k_stream_level(const SliderTables* tabs, PersistArgs p, int L) {
     a.tile_ctr = &ctl->tile_ctr[L];
     a.err = &ctl->err;
     a.out_count = &ctl->out_count[L];
-    a.tile_parents = level_tile_parents(n, gridDim.x, Shape::TP);
+    a.tile_parents = level_tile_parents(n, gridDim.x, TP);
     if (MODE == kLeaf) {
         a.out = nullptr;
         a.out_root = nullptr;
@@ -111,6 +112,23 @@ void k_stream_level(const SliderTables* tabs, PersistArgs p, int L) {
     expand_level(sl, a, kWrite, s);
 }
 
+template <int B, int MODE, int SH, int WV>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(StreamWaves<WV>::v)))
+void k_stream_level(const SliderTables* tabs, PersistArgs p, int L) {
+    using Shape = PCfg<B, StreamShape<SH>::v>;
+    __shared__ TileLds<Shape::TP, Shape::CAP, kWG> s;
+    stream_level<MODE>([&] { return SlOf<B>::make(tabs); }, p, L, s);
+}
+
+// The byte backend's LDS form (level_dev.h): one 768-thread workgroup per CU,
+// tables staged in its LDS.
+template <int MODE>
+__global__ __launch_bounds__(kByteLdsWG) void k_stream_level_byte_lds(const SliderTables* tabs, PersistArgs p, int L) {
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    __shared__ TileLds<kByteLdsTP, kByteLdsCap, kByteLdsWG> s;
+    stream_level<MODE>([&] { return stage_byte_tables(tabs, dyn_lds); }, p, L, s);
+}
+
 // ---- launch plumbing --------------------------------------------------------
 template <int B, int MODE>
 static void* level_fn(int v) {
@@ -130,6 +148,9 @@ static void* level_fn_mode(int mode, int v) {
     return mode == kWrite ? level_fn<B, kWrite>(v) : level_fn<B, kLeaf>(v);
 }
 static void* pick_stream_level(int backend, int mode, int v) {
+    if (backend == kByte && v == kStreamByteLds)
+        return mode == kWrite ? reinterpret_cast<void*>(&k_stream_level_byte_lds<kWrite>)
+                              : reinterpret_cast<void*>(&k_stream_level_byte_lds<kLeaf>);
     switch (backend) {
     case kKS: return level_fn_mode<kKS>(mode, v);
     case kPlain: return level_fn_mode<kPlain>(mode, v);
@@ -146,8 +167,17 @@ static void* pick_stream_roots(int backend) {
     }
 }
 
+// Workgroup size and dynamic LDS of a level kernel form.
+static int stream_wg(int backend, int variant) {
+    return backend == kByte && variant == kStreamByteLds ? kByteLdsWG : kWG;
+}
+static size_t stream_dyn(int backend, int variant) {
+    return backend == kByte && variant == kStreamByteLds ? kByteLds : 0;
+}
+
 int stream_grid(int backend, int mode, int variant, int n_cu) {
-    return blocks_per_cu(pick_stream_level(backend, mode, variant)) * n_cu;
+    return blocks_per_cu(pick_stream_level(backend, mode, variant), stream_wg(backend, variant),
+                         stream_dyn(backend, variant)) * n_cu;
 }
 
 hipError_t launch_stream(int backend, const DevTables& dt, const PersistArgs& a, const int grid[2],
@@ -163,8 +193,11 @@ hipError_t launch_stream(int backend, const DevTables& dt, const PersistArgs& a,
     for (int L = 0; L <= a.depth - 2; L++) {
         const int mode = (L == a.depth - 2) ? kLeaf : kWrite;
         void* kargs[] = {(void*)&t, (void*)&args, (void*)&L};
-        const hipError_t e = hipLaunchKernel(pick_stream_level(backend, mode, variant[mode]), dim3(grid[mode]),
-                                             dim3(kWG), kargs, 0, s);
+        // (a form with dynamic LDS had the attribute set when its grid was sized)
+        void* f = pick_stream_level(backend, mode, variant[mode]);
+        const size_t dyn = stream_dyn(backend, variant[mode]);
+        const hipError_t e = hipLaunchKernel(f, dim3(grid[mode]), dim3(stream_wg(backend, variant[mode])), kargs,
+                                             dyn, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

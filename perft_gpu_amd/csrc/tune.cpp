@@ -30,8 +30,11 @@ void Engine::apply_tuning() {
     for (int b = 0; b < 4; b++) {
         Tune& t = tune[b];
         if (t.p_variant < 0 || t.p_variant >= kPersistVariants) t.p_variant = 0;
-        if (t.s_leaf < 0 || t.s_leaf >= kStreamVariants) t.s_leaf = kStreamLeafDefault;
-        if (t.s_write < 0 || t.s_write >= kStreamWriteVariants) t.s_write = kStreamWriteDefault;
+        const bool lds_form_ok = b == kByte;
+        if (t.s_leaf < 0 || (t.s_leaf >= kStreamVariants && !(lds_form_ok && t.s_leaf == kStreamByteLds)))
+            t.s_leaf = kStreamLeafDefault;
+        if (t.s_write < 0 || (t.s_write >= kStreamWriteVariants && !(lds_form_ok && t.s_write == kStreamByteLds)))
+            t.s_write = kStreamWriteDefault;
         if (t.tt_chunk_pct < 10 || t.tt_chunk_pct > 90) t.tt_chunk_pct = 50;
         const int pmax = persistent_max_bpc(b, t.p_variant);
         const int pb = (t.p_bpc > 0 && t.p_bpc < pmax) ? t.p_bpc : pmax;
@@ -189,8 +192,10 @@ pf_status Engine::autotune(const pf_pos48* pos, int n, int d, int what) {
         if (s == PF_OK) {  // the stream-ordered form: its leaf kernel's forms, then its write kernel's
             double stream_ms = 1e30;
             for (int pass = 0; pass < 2 && s == PF_OK; pass++) {
-                const int nsv = pass == 0 ? kStreamVariants : kStreamWriteVariants;
-                for (int v = 0; v < nsv; v++) {
+                // the forms of this mode, and for the byte backend its LDS form
+                const int nsv = (pass == 0 ? kStreamVariants : kStreamWriteVariants) + (b == kByte ? 1 : 0);
+                for (int i = 0; i < nsv; i++) {
+                    const int v = i < (pass == 0 ? kStreamVariants : kStreamWriteVariants) ? i : kStreamByteLds;
                     if (pass == 0) tune[b].s_leaf = v; else tune[b].s_write = v;
                     apply_tuning();
                     double ms = 0;

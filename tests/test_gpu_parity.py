@@ -79,6 +79,30 @@ def test_c2_backends_vs_golden(pg, engine, backend):
         engine.set_backend("ks")
 
 
+def test_byte_backend_lds_form_vs_golden(pg, tmp_path):
+    """the byte backend's other stream-ordered form, its tables staged in each
+    workgroup's LDS, selected through a tune file for both level kinds; the
+    tuner may also choose it, and its counts must equal the cache form's"""
+    g = _golden()["c2"]
+    pos = _rows(pg, g["positions"])
+    path = tmp_path / "lds.txt"
+    path.write_text("perft_gpu_amd tuning 1\nbackend byte stream_leaf 6 stream_write 6\n")
+    eng = pg.Engine(backend="byte", arena_bytes=8 << 30, tune_file=str(path))
+    try:
+        assert [int(x) for x in eng.perft_batch(pos, 5, mode=pg.MODE_STREAM)] == g["perft"]["5"]
+        assert [int(x) for x in eng.perft_batch(pos[:32], 4, mode=pg.MODE_STREAM)] == g["perft"]["4"][:32]
+    finally:
+        eng.close()
+    # the same form number means nothing to the other backends: a tune file
+    # naming it for them falls back to their defaults
+    path.write_text("perft_gpu_amd tuning 1\nbackend ks stream_leaf 6 stream_write 6\n")
+    eng = pg.Engine(backend="ks", arena_bytes=8 << 30, tune_file=str(path))
+    try:
+        assert [int(x) for x in eng.perft_batch(pos[:32], 4, mode=pg.MODE_STREAM)] == g["perft"]["4"][:32]
+    finally:
+        eng.close()
+
+
 def test_c3_c5_positions_vs_golden(pg, engine):
     g = _golden()
     c3 = _rows(pg, g["c3"]["positions"])
