@@ -23,7 +23,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
           "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
 LIB_SOURCES = ["kernels.hip", "stream.hip", "engine.cpp", "engine_tt.cpp", "tables.cpp", "capi.cpp", "probes.hip", "serve.cpp",
                "tt.hip", "tune.cpp", "records.cpp", "tt_file.cpp", "noalloc.hip"]
-HEADERS = ["level_dev.h", "board.h", "sliders.h", "movegen.h", "kernels.h", "engine.h", "tables.h",
+HEADERS = ["level_dev.h", "wave.h", "board.h", "sliders.h", "movegen.h", "kernels.h", "engine.h", "tables.h",
            "fixed_shift_magics.inc", "serve.h", "tt.h"]
 
 

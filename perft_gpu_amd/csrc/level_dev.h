@@ -6,6 +6,7 @@
 
 #include "backends_dev.h"
 #include "kernels.h"
+#include "wave.h"
 
 namespace pf {
 
@@ -15,20 +16,6 @@ namespace pf {
 // 146 KiB of tables staged in the workgroup's LDS (below).
 constexpr int kWG = 256;
 
-__device__ __forceinline__ u32 wave_incl_scan(u32 v, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const u32 o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
-}
-
-__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
 
 // The byte backend's LDS form: the 148 KB lookup blob and the 4 KB of magic
 // entries are copied into LDS once per workgroup (the workgroup is persistent,
@@ -116,7 +103,7 @@ __device__ __forceinline__ void expand_level(const SL& sl, const LevelArgs& a, c
             }
             s.sum[tid] = 0;
         }
-        const u32 incl = wave_incl_scan(cnt, lane);
+        const u32 incl = wave_incl_scan(cnt);
         if (lane == 63) s.wtot[wid] = incl;
         __syncthreads();
         u32 woff = 0, tot = 0;
@@ -212,7 +199,8 @@ struct LevelWaves {
 // Workgroups of one of these kernels that fit a CU (a kernel with dynamic LDS
 // has it enabled first).
 static inline int blocks_per_cu(void* f, int wg = kWG, size_t dyn = 0) {
-    if (dyn) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    if (dyn && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess)
+        (void)hipGetLastError();  // the occupancy query below then reports what fits
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, wg, dyn) != hipSuccess || per_cu < 1) {
         (void)hipGetLastError();  // a failed query must not surface as a later call's error

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include "tt_kernels.h"
+#include "wave.h"
 
 namespace pf {
 
@@ -32,21 +33,6 @@ constexpr int kTTWaves = 4;
 // 40 B per lane for Kogge-Stone): C3 perft 8 0.0879 -> 0.0868 s and C5
 // 2.189 -> 2.162 s against four waves without spills.
 constexpr int kTTLeafWaves = 5;
-
-__device__ __forceinline__ u32 t_wave_incl_scan(u32 v, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const u32 o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
-}
-
-__device__ __forceinline__ unsigned long long t_wave_sum64(unsigned long long v) {
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
 
 // Adds each lane's v to its *p with one global atomic per run of consecutive
 // lanes that hold the same address.  A level's children are written in their
@@ -306,7 +292,7 @@ __device__ __forceinline__ void write_window(const SL& sl, const TTArgs& a, cons
             // the wave's children are consecutive: at most two blocks
             const u64 b0 = first >> kBsumShift;
             const u64 part = keep ? (((pos >> kBsumShift) == b0) ? (u64)cc : (u64)cc << 32) : 0;
-            const u64 t = t_wave_sum64(part);
+            const u64 t = wave_sum64(part);
             if (lane == 0) {
                 if ((u32)t) atomicAdd(&a.out_bsum[b0], (u32)t);
                 if (t >> 32) atomicAdd(&a.out_bsum[b0 + 1], (u32)(t >> 32));
@@ -343,7 +329,7 @@ __device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTa
             s.sum[tid] = 0;
             s.hit[tid] = 0;
         }
-        const u32 incl = t_wave_incl_scan(cnt, lane);
+        const u32 incl = wave_incl_scan(cnt);
         if (lane == 63) s.wtot[wid] = incl;
         __syncthreads();
         u32 woff = 0, tot = 0;
@@ -388,7 +374,7 @@ __device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTa
             tsum = s.sum[tid];
         }
         if (a.total) {
-            const unsigned long long v = t_wave_sum64(tsum);
+            const unsigned long long v = wave_sum64(tsum);
             if (lane == 0) s.red[wid] = v;
         }
         __syncthreads();
@@ -489,7 +475,7 @@ __global__ __launch_bounds__(256) void k_upsweep(UpSeg u, uint4* tt, u64 mask, u
     if (i < u.n + u.ndup) st = upsweep_item(u, i, tt, mask, &dst, &v);
     wave_run_add(dst, v);
     if (stores) {
-        st = t_wave_sum64(st);
+        st = wave_sum64(st);
         if ((threadIdx.x & 63) == 0 && st) atomicAdd(stores, st);
     }
 }
@@ -577,7 +563,7 @@ __global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 ma
         wave_run_add(dst, v);
     }
     if (t->stores) {
-        stores = t_wave_sum64(stores);
+        stores = wave_sum64(stores);
         if ((threadIdx.x & 63) == 0 && stores) atomicAdd(t->stores, stores);
     }
 }
@@ -735,7 +721,7 @@ __global__ __launch_bounds__(256) void k_plan_write(TTPlanCtl* c, StepTable* t, 
         const TTLevelRec& top = c->p.top;
         unsigned long long v = 0;
         for (u64 i = threadIdx.x; i < top.n; i += 256) v += (u64)meta_count(top.b[i]);
-        v = t_wave_sum64(v);
+        v = wave_sum64(v);
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
         __syncthreads();
     }
