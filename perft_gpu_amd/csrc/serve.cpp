@@ -191,6 +191,23 @@ bool Server::region_alloc(u64 bytes, u64* off) {
     return false;
 }
 
+// With nothing waiting for admission, a job whose level would chunk takes up
+// to `extra` bytes from the free block that starts where its region ends, if
+// there is one: the last deep requests of a burst then run in a few large
+// chunks instead of a long tail of small steps.  The rest of the block stays
+// free for later arrivals; what was taken returns with the region when the
+// job finishes.
+bool Server::grow_region(Job& j, u64 extra) {
+    auto it = free_mem_.find(j.base + j.cap);
+    if (it == free_mem_.end() || !extra) return false;
+    const u64 take_b = std::min(it->second, (extra + kAl - 1) & ~(kAl - 1));
+    const u64 rest = it->second - take_b, at = it->first + take_b;
+    free_mem_.erase(it);
+    if (rest) free_mem_[at] = rest;
+    j.cap += take_b;
+    return true;
+}
+
 void Server::region_free(u64 off, u64 bytes) {
     auto it = free_mem_.emplace(off, bytes).first;
     auto nx = std::next(it);
@@ -420,9 +437,16 @@ bool Server::plan(Job& j) {
                 // per 1024 children (charged as 1 B per child) and leave a quarter
                 const bool leaf_next = r - 1 == 2;
                 const u64 node_b = kTNode + (dd ? 32 : 0) + (leaf_next ? 0 : 1);
-                const u64 free_b = j.cap - j.top;
+                u64 free_b = j.cap - j.top;
                 u64 cap;
-                const u64 room = leaf_next ? free_b : free_b / 2 + free_b / 4;
+                u64 room = leaf_next ? free_b : free_b / 2 + free_b / 4;
+                const u64 need = f.n_next * node_b + 11 * kAl;  // the whole level, unchunked
+                const u64 want_free = leaf_next ? need : need / 3 * 4 + kAl;
+                if (queue_idle_ && (f.chunking || need > room) && want_free > free_b &&
+                    grow_region(j, want_free - free_b)) {
+                    free_b = j.cap - j.top;
+                    room = leaf_next ? free_b : free_b / 2 + free_b / 4;
+                }
                 if (!f.chunking && f.n_next * node_b + 11 * kAl <= room) {
                     cap = f.n_next;
                 } else {
@@ -554,6 +578,7 @@ void Server::run() {
                 jobs_[(size_t)slot] = std::move(jb);
                 admitted.push_back(slot);
             }
+            queue_idle_ = queue_.empty();
         }
         StepTable& t = *h_tab_;
         t.nseg = t.nup = t.nroot = 0;

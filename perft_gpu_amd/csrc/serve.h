@@ -56,6 +56,7 @@ class Server {
     u64 predict_bytes(const pf_pos48& pos, int depth) const;
     bool region_alloc(u64 bytes, u64* off);
     void region_free(u64 off, u64 bytes);
+    bool grow_region(Job& j, u64 extra);
     bool plan(Job& j);
     int chain_up(Job& j, int k);
     bool chunk(Job& j, Frame& f, u64 budget, u64* cap);
@@ -72,6 +73,7 @@ class Server {
     std::mutex mu_;
     std::condition_variable cv_;
     bool stop_ = false, running_ = false;
+    bool queue_idle_ = false;  // nothing was waiting for admission this step (server thread)
     int graph_status_ = 0;  // the serving thread's graph capture: 1 pending, else a pf_status
     std::deque<Job> queue_;        // submitted, not yet admitted
     std::vector<pf_result> done_;  // finished, not yet polled
