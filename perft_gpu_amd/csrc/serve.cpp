@@ -655,7 +655,7 @@ void Server::run() {
         u32 acc_t = 0;
         for (u32 i = 0; i < t.nseg; i++) {
             t.tile_prefix[i] = acc_t;
-            acc_t += (t.seg[i].n_in + 127) / 128;
+            acc_t += (t.seg[i].n_in + kTTTileParents - 1) / kTTTileParents;
         }
         t.tile_prefix[t.nseg] = acc_t;
         // upsweep segments laid out round by round

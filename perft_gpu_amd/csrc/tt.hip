@@ -23,7 +23,10 @@
 namespace pf {
 
 constexpr int kTWG = 256;
-constexpr int kTTP = 128;   // parents per tile
+// Parents per tile: 112 parents have about 3,600 moves (32.5 per node on
+// average), so a tile's moves mostly fit one 4096-entry window; 128 had
+// about 4,200 and spilled a few dozen into a second window almost every tile.
+constexpr int kTTP = kTTTileParents;
 constexpr int kTCAP = 4096; // LDS move-list window
 // The write kernels wait on random table and duplicate-claim traffic most of
 // the time: ask for at least this many resident waves per SIMD (register
@@ -176,7 +179,7 @@ __device__ __forceinline__ void write_window(const SL& sl, const TTArgs& a, cons
         bool probed = false;
         if (j < m) {
             const u32 e = s.mv[j];
-            const u32 par = (e >> 16) & (kTTP - 1), mvv = e & 0xFFFFu;
+            const u32 par = (e >> 16) & 0xFFu, mvv = e & 0xFFFFu;
             const Board pb = s.b[par];
             Board c = make_move(pb, mvv);
             if (a.ep_always) c = ep_on_every_double_push(c, mvv);
@@ -236,7 +239,7 @@ __device__ __forceinline__ void write_window(const SL& sl, const TTArgs& a, cons
         if (j < m) {
             code = s.code[j];
             const u32 e = s.mv[j];
-            par = (e >> 16) & (kTTP - 1);
+            par = (e >> 16) & 0xFFu;
             mvv = e & 0xFFFFu;
         }
         const u32 res = code & 3u;
@@ -356,7 +359,7 @@ __device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTa
                     const u32 j = jb + lane;
                     if (j < m) {
                         const u32 e = s.mv[j];
-                        const u32 par = (e >> 16) & (kTTP - 1);
+                        const u32 par = (e >> 16) & 0xFFu;
                         const u32 cc = count_moves(make_move(s.b[par], e & 0xFFFFu), sl);
                         atomicAdd(&s.sum[par], cc);
                     }

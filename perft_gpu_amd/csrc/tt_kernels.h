@@ -13,6 +13,10 @@ namespace pf {
 // what a random 32-byte probe plus its share of the store costs.
 constexpr int kMinTTDepth = 3;
 
+// Parents per tile of the table kernels (a tile's move list is one LDS
+// window of 4096 entries); the serving step's tile prefix counts with it too.
+constexpr int kTTTileParents = 112;
+
 // In-level duplicate detection, at the moment a child is made (open
 // addressing, linear probing).  `key` is the child's second Zobrist key
 // (0 = empty slot), claimed by compare-and-swap; the first child of a
