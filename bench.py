@@ -655,10 +655,10 @@ def main():
         eng.set_backend(args.backend)
         # the byte backend's other form: its tables staged in each workgroup's LDS
         with tempfile.TemporaryDirectory() as td:
-            tf = os.path.join(td, "byte_lds.txt")
-            with open(tf, "w") as f:
+            lds_tf = os.path.join(td, "byte_lds.txt")
+            with open(lds_tf, "w") as f:
                 f.write("perft_gpu_amd tuning 1\nbackend byte stream_leaf 6 stream_write 6\n")
-            e2 = pg.Engine(backend="byte", arena_bytes=16 << 30, tune_file=tf)
+            e2 = pg.Engine(backend="byte", arena_bytes=16 << 30, tune_file=lds_tf)
             try:
                 e2.perft_batch(pos, depth, mode=pg.MODE_STREAM)
                 t = time.perf_counter()

@@ -191,8 +191,9 @@ bool Server::region_alloc(u64 bytes, u64* off) {
     return false;
 }
 
-// With nothing waiting for admission, a job whose level would chunk takes up
-// to `extra` bytes from the free block that starts where its region ends, if
+// With nothing waiting for admission (and regions not fixed by job_bytes), a
+// job whose level would chunk takes up to `extra` bytes from the free block
+// that starts where its region ends, if
 // there is one: the last deep requests of a burst then run in a few large
 // chunks instead of a long tail of small steps.  The rest of the block stays
 // free for later arrivals; what was taken returns with the region when the
@@ -442,7 +443,7 @@ bool Server::plan(Job& j) {
                 u64 room = leaf_next ? free_b : free_b / 2 + free_b / 4;
                 const u64 need = f.n_next * node_b + 11 * kAl;  // the whole level, unchunked
                 const u64 want_free = leaf_next ? need : need / 3 * 4 + kAl;
-                if (queue_idle_ && (f.chunking || need > room) && want_free > free_b &&
+                if (!job_bytes_ && queue_idle_ && (f.chunking || need > room) && want_free > free_b &&
                     grow_region(j, want_free - free_b)) {
                     free_b = j.cap - j.top;
                     room = leaf_next ? free_b : free_b / 2 + free_b / 4;
