@@ -137,3 +137,32 @@ def test_serving_no_table_leaf_parent_chunks(pg):
         eng.serve_stop()
     finally:
         eng.close()
+
+
+def test_serving_job_grows_into_free_memory(pg):
+    """a 256 MiB arena caps each job's predicted region at 64 MiB, where the
+    ply-4 level of a depth-6 request (about 1.5e6 nodes) does not fit.  Alone
+    (nothing waiting for admission) the job takes the free memory after its
+    region and runs the level whole, in fewer steps than the same request in a
+    fixed 64 MiB region, which chunks it; both equal the oracle's count"""
+    with open(os.path.join(GOLDEN, "workloads.json")) as f:
+        want = json.load(f)["c5"]["perft_first8_d6"][:2]
+    from perft_gpu_amd import workloads as W
+    pos = W.positions(W.C5)[:2]
+    steps = {}
+    for name, job_bytes in (("grows", 0), ("fixed", 64 << 20)):
+        eng = pg.Engine(backend="ks", arena_bytes=256 << 20, tt_bytes=64 << 20, zobrist_seed=5)
+        try:
+            eng.serve_start(max_jobs=4, job_bytes=job_bytes)
+            got = []
+            for i in range(2):  # one at a time, so that nothing waits while it runs
+                eng.submit(pos[i], 6, i)
+                r = _drain(eng, 1, timeout=300.0)[i]
+                assert r["status"] == 0
+                got.append(r["nodes"])
+            steps[name] = eng.stats()["levels_run"]
+            eng.serve_stop()
+        finally:
+            eng.close()
+        assert got == want, name
+    assert steps["grows"] < steps["fixed"], steps
