@@ -79,7 +79,6 @@ pf_status Server::start() {
     free_slots_.clear();
     for (int i = n - 1; i >= 0; i--) free_slots_.push_back(i);
     grid_ = step_grid(e_.backend, e_.n_cu);
-    leaf_grid_ = step_max_bpc(e_.backend, 1) * e_.n_cu;
     up_grid_ = e_.n_cu * 4;
     stop_ = false;
     // the serving thread captures the step graph itself and then replays it,
@@ -107,7 +106,7 @@ pf_status Server::build_graph() {
     hipError_t e = hipMemcpyAsync(d_tab_, h_tab_, sizeof(StepTable), hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = launch_step(e_.backend, e_.d_tab, e_.d_z, d_tab_, e_.d_tt, e_.tt_buckets ? e_.tt_mask_word() : 0, grid_,
-                        leaf_grid_, up_grid_, st);
+                        up_grid_, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(&h_tab_->out[0], &d_tab_->out[0], sizeof(h_tab_->out) + sizeof(h_tab_->rout),
                            hipMemcpyDeviceToHost, st);
@@ -125,6 +124,19 @@ pf_status Server::build_graph() {
     graph_ = g;
     if (hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0) != hipSuccess) return PF_EDEVICE;
     return PF_OK;
+}
+
+// Workgroups of the mixed step launch that start on the write tiles: a share
+// of the grid proportional to the write tiles' part of the step's work, a
+// write tile costing about as much as kWriteTileCost leaf tiles (C5's 4096
+// positions at perft 6 write 1.1e10 children/s and count 5.1e10 in leaf
+// passes).  Whichever queue empties first, its workgroups move to the other.
+static constexpr double kWriteTileCost = 4.6;
+u32 Server::mix_write_wgs(u32 write_tiles, u32 leaf_tiles) const {
+    if (!write_tiles) return 0;
+    const double w = kWriteTileCost * write_tiles;
+    const u32 n = (u32)((double)grid_ * w / (w + leaf_tiles) + 0.5);
+    return std::max(1u, std::min(n, (u32)grid_));
 }
 
 void Server::stop() {
@@ -639,9 +651,9 @@ void Server::run() {
             active.clear();
             continue;
         }
-        // write segments first, then leaf segments (two launches); a segment
-        // keeps its outcome slot (its out pointers), so jobs find their
-        // outcome where they planned it
+        // write segments first, then leaf segments (two tile queues of one
+        // launch); a segment keeps its outcome slot (its out pointers), so
+        // jobs find their outcome where they planned it
         {
             u32 nw = 0;
             for (u32 i = 0; i < t.nseg; i++)
@@ -659,6 +671,7 @@ void Server::run() {
             acc_t += (t.seg[i].n_in + kTTTileParents - 1) / kTTTileParents;
         }
         t.tile_prefix[t.nseg] = acc_t;
+        t.mix_write_wgs = mix_write_wgs(t.tile_prefix[t.nseg_write], acc_t - t.tile_prefix[t.nseg_write]);
         // upsweep segments laid out round by round
         u32 acc_u = 0;
         for (int r = 0; r < kUpRounds; r++) {

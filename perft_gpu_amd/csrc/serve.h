@@ -65,6 +65,7 @@ class Server {
     char* take(Job& j, u64 bytes);
     bool alloc_level(Job& j, u64 n, Engine::TLevel* lv);
     pf_status build_graph();
+    u32 mix_write_wgs(u32 write_tiles, u32 leaf_tiles) const;
 
     Engine& e_;
     int max_jobs_;
@@ -95,7 +96,7 @@ class Server {
     unsigned long long* h_acc_ = nullptr;
     unsigned long long* h_stats_ = nullptr;  // the table counters, copied back by every step
     u64 probes_ = 0, hits_ = 0, root_probes_ = 0, root_hits_ = 0;
-    int grid_ = 0, leaf_grid_ = 0, up_grid_ = 0;
+    int grid_ = 0, up_grid_ = 0;
     std::vector<TTArgs> seg_tmp_;  // reordering buffer of a step's segments
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;

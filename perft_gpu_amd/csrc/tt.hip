@@ -36,6 +36,12 @@ constexpr int kTTWaves = 4;
 // 40 B per lane for Kogge-Stone): C3 perft 8 0.0879 -> 0.0868 s and C5
 // 2.189 -> 2.162 s against four waves without spills.
 constexpr int kTTLeafWaves = 5;
+// The serving step's mixed launch (write and leaf tiles at once) is built for
+// five too: 96 VGPRs, 96 B per lane of spills, five workgroups per CU.  On the
+// 6,018-request serving stream submitted at once: 7.60e12 leaves/s for the
+// write and leaf launches one after the other, 7.96e12 mixed at four waves,
+// 8.04-8.12e12 at five, 8.36-8.40e12 once a tile is taken with one atomic.
+constexpr int kTTMixedWaves = 5;
 
 // Adds each lane's v to its *p with one global atomic per run of consecutive
 // lanes that hold the same address.  A level's children are written in their
@@ -499,13 +505,13 @@ __device__ __forceinline__ TTArgs uniform_args(const TTArgs& src) {
     return a;
 }
 
-// Tiles of all expand segments form one queue (StepTable::tile_prefix); a
-// block copies its segment's arguments to LDS and runs the tile.  Segments of
-// different jobs touch disjoint memory except the lockless table.
-// W: the waves per SIMD it is built for; M: kWrite runs the table's write
-// segments [0, nseg_write), kLeaf its leaf segments [nseg_write, nseg), each
-// without the other mode's code (as in k_expand_tt); -1 runs every segment.
-template <int B, int W = kTTWaves, int M = -1>
+// Tiles of the step's expand segments form a queue (StepTable::tile_prefix);
+// a block copies its segment's arguments to LDS and runs the tile.  Segments
+// of different jobs touch disjoint memory except the lockless table.
+// W: the waves per SIMD it is built for; M = kWrite runs the table's write
+// segments [0, nseg_write) and M = kLeaf its leaf segments [nseg_write,
+// nseg), each without the other mode's code (as in k_expand_tt).
+template <int B, int W, int M>
 __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void k_step(const SliderTables* tabs, const ZTables* zt, StepTable* t) {
     __shared__ TTLds s;
     __shared__ TTArgs sa;
@@ -539,7 +545,67 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
         }
         __syncthreads();
         TTArgs a = uniform_args(sa);
-        if (M >= 0) a.mode = M;
+        a.mode = M;
+        tt_tile(sl, a, g - t->tile_prefix[seg], zt, s);
+    }
+}
+
+// The serving step's write and leaf segments in one launch, overlapped:
+// workgroups [0, mix_write_wgs) start on the write tiles, whose waves mostly
+// wait on table and duplicate-table accesses, the others on the leaf tiles,
+// which keep the VALU busy; a workgroup whose queue is empty moves to the
+// other one, and leaves when that is empty too.  The queues are the write
+// segments [0, nseg_write) with tile_ctr and the leaf segments [nseg_write,
+// nseg) with tile_ctr2; their bounds are read once, before the loop, so that
+// taking a tile is one atomic (small steps are latency-bound).
+template <int B, int W>
+__global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void k_step_mixed(const SliderTables* tabs, const ZTables* zt, StepTable* t) {
+    __shared__ TTLds s;
+    __shared__ TTArgs sa;
+    __shared__ u32 s_tile, s_seg;
+    const auto sl = SlOf<B>::make(tabs);
+    const u32 nw = t->nseg_write, ns = t->nseg;
+    const u32 wt = t->tile_prefix[nw], lt = t->tile_prefix[ns];  // write tiles [0, wt), leaf tiles [wt, lt)
+    bool leaf = blockIdx.x >= t->mix_write_wgs;  // thread 0's queue
+    bool moved = false;
+    u32 cur = 0xFFFFFFFFu;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            u32 g = 0xFFFFFFFFu;
+            for (;;) {
+                const u32 first = leaf ? wt : 0u, total = leaf ? lt : wt;
+                if (first < total) {
+                    const u32 k = first + atomicAdd(leaf ? &t->tile_ctr2 : &t->tile_ctr, 1u);
+                    if (k < total) {
+                        g = k;
+                        break;
+                    }
+                }
+                if (moved) break;
+                leaf = !leaf;
+                moved = true;
+            }
+            u32 seg = 0;
+            if (g != 0xFFFFFFFFu) {
+                u32 lo = leaf ? nw : 0u, hi = leaf ? ns : nw;  // largest seg with tile_prefix[seg] <= g
+                while (hi - lo > 1) {
+                    const u32 mid = (lo + hi) >> 1;
+                    if (t->tile_prefix[mid] <= g) lo = mid; else hi = mid;
+                }
+                seg = lo;
+            }
+            s_tile = g;
+            s_seg = seg;
+        }
+        __syncthreads();
+        const u32 g = s_tile, seg = s_seg;
+        if (g == 0xFFFFFFFFu) break;
+        if (seg != cur) {
+            if (threadIdx.x == 0) sa = t->seg[seg];
+            cur = seg;
+        }
+        __syncthreads();
+        const TTArgs a = uniform_args(sa);  // the segment's own mode
         tt_tile(sl, a, g - t->tile_prefix[seg], zt, s);
     }
 }
@@ -631,37 +697,42 @@ int step_max_bpc(int backend, int leaf) {
     return per_cu;
 }
 
-int step_grid(int backend, int n_cu) { return step_max_bpc(backend, 0) * n_cu; }
+// The serving step's mixed kernel: its most resident workgroups per CU x CUs.
+int step_grid(int backend, int n_cu) {
+    const void* f = backend == kKS ? (const void*)&k_step_mixed<kKS, kTTMixedWaves>
+                    : backend == kPlain ? (const void*)&k_step_mixed<kPlain, kTTMixedWaves>
+                    : backend == kFancy ? (const void*)&k_step_mixed<kFancy, kTTMixedWaves>
+                                        : (const void*)&k_step_mixed<kByte, kTTMixedWaves>;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kTWG, 0) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    return per_cu * n_cu;
+}
 
-// One step: start (roots, fills), expand/leaf segments, kUpRounds upsweep
-// rounds.  up_grid * 256 must cover kMaxRoots (the roots are handled by the
-// first threads of the start kernel).
+// One step: start (roots, fills), expand/leaf segments in one mixed launch,
+// kUpRounds upsweep rounds.  up_grid * 256 must cover kMaxRoots (the roots
+// are handled by the first threads of the start kernel).
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
-                       int grid, int leaf_grid, int up_grid, hipStream_t s) {
+                       int grid, int up_grid, hipStream_t s) {
     if (up_grid * 256 < kMaxRoots) return hipErrorInvalidValue;
     switch (backend) {
-    case kKS: hipLaunchKernelGGL(k_pre_step<kKS>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask); break;
-    case kPlain: hipLaunchKernelGGL(k_pre_step<kPlain>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask); break;
-    case kFancy: hipLaunchKernelGGL(k_pre_step<kFancy>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask); break;
-    default: hipLaunchKernelGGL(k_pre_step<kByte>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask); break;
-    }
-    // the write segments, then the leaf segments (five-wave leaf build)
-    switch (backend) {
     case kKS:
-        hipLaunchKernelGGL((k_step<kKS, kTTWaves, kWrite>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
-        hipLaunchKernelGGL((k_step<kKS, kTTLeafWaves, kLeaf>), dim3(leaf_grid), dim3(kTWG), 0, s, tabs, zt, t);
+        hipLaunchKernelGGL(k_pre_step<kKS>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask);
+        hipLaunchKernelGGL((k_step_mixed<kKS, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
         break;
     case kPlain:
-        hipLaunchKernelGGL((k_step<kPlain, kTTWaves, kWrite>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
-        hipLaunchKernelGGL((k_step<kPlain, kTTLeafWaves, kLeaf>), dim3(leaf_grid), dim3(kTWG), 0, s, tabs, zt, t);
+        hipLaunchKernelGGL(k_pre_step<kPlain>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask);
+        hipLaunchKernelGGL((k_step_mixed<kPlain, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
         break;
     case kFancy:
-        hipLaunchKernelGGL((k_step<kFancy, kTTWaves, kWrite>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
-        hipLaunchKernelGGL((k_step<kFancy, kTTLeafWaves, kLeaf>), dim3(leaf_grid), dim3(kTWG), 0, s, tabs, zt, t);
+        hipLaunchKernelGGL(k_pre_step<kFancy>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask);
+        hipLaunchKernelGGL((k_step_mixed<kFancy, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
         break;
     default:
-        hipLaunchKernelGGL((k_step<kByte, kTTWaves, kWrite>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
-        hipLaunchKernelGGL((k_step<kByte, kTTLeafWaves, kLeaf>), dim3(leaf_grid), dim3(kTWG), 0, s, tabs, zt, t);
+        hipLaunchKernelGGL(k_pre_step<kByte>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask);
+        hipLaunchKernelGGL((k_step_mixed<kByte, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
         break;
     }
     for (int r = 0; r < kUpRounds; r++) hipLaunchKernelGGL(k_up_step, dim3(up_grid), dim3(256), 0, s, t, tt, mask, r);

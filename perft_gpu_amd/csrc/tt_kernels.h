@@ -127,8 +127,11 @@ constexpr int kUpRounds = 4;
 struct StepTable {
     u32 nseg, nup, nroot, tile_ctr;
     // segments [0, nseg_write) are write levels, [nseg_write, nseg) leaf
-    // passes: each kind runs in a launch of its own (its own tile counter)
+    // passes, each kind with its own tile counter (tile_ctr, tile_ctr2)
     u32 nseg_write, tile_ctr2;
+    // the serving step's mixed launch: workgroups [0, mix_write_wgs) start on
+    // the write tiles, the others on the leaf tiles
+    u32 mix_write_wgs, pad0;
     unsigned long long* stores;    // table stores of the upsweeps are counted here (null: not counted)
     u32 tile_prefix[kMaxSegs + 1];
     u32 up_prefix[kMaxUps + 1];    // item prefix over all upsweep segments
@@ -195,10 +198,10 @@ hipError_t capture_plan_pass(int backend, const SliderTables* tabs, const ZTable
                              u64 mask, unsigned long long* ttstats, int r, int write_grid, int leaf_grid,
                              int up_grid, hipStream_t s);
 
-int step_grid(int backend, int n_cu);
+int step_grid(int backend, int n_cu);       // workgroups of the serving step's mixed launch
 int step_max_bpc(int backend, int leaf);  // the planned pass's write (0) / leaf (1) step kernel
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
-                       int grid, int leaf_grid, int up_grid, hipStream_t s);
+                       int grid, int up_grid, hipStream_t s);
 int expand_tt_max_bpc(int backend);   // most resident workgroups per CU of the TT write-level kernel
 int expand_tt_leaf_bpc(int backend);  // the same for the TT leaf-level kernel
 hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables* zt, const TTArgs& a, int grid,
