@@ -127,16 +127,18 @@ pf_status Server::build_graph() {
 }
 
 // Workgroups of the mixed step launch that start on the write tiles: a share
-// of the grid proportional to the write tiles' part of the step's work, a
-// write tile costing about as much as kWriteTileCost leaf tiles (C5's 4096
+// of the workgroups that get a tile (those past the step's tile count leave
+// at once) proportional to the write tiles' part of the step's work, a write
+// tile costing about as much as kWriteTileCost leaf tiles (C5's 4096
 // positions at perft 6 write 1.1e10 children/s and count 5.1e10 in leaf
 // passes).  Whichever queue empties first, its workgroups move to the other.
 static constexpr double kWriteTileCost = 4.6;
 u32 Server::mix_write_wgs(u32 write_tiles, u32 leaf_tiles) const {
     if (!write_tiles) return 0;
+    const u32 active = std::min<u32>((u32)grid_, write_tiles + leaf_tiles);
     const double w = kWriteTileCost * write_tiles;
-    const u32 n = (u32)((double)grid_ * w / (w + leaf_tiles) + 0.5);
-    return std::max(1u, std::min(n, (u32)grid_));
+    const u32 n = (u32)((double)active * w / (w + leaf_tiles) + 0.5);
+    return std::max(1u, std::min(n, active));
 }
 
 void Server::stop() {

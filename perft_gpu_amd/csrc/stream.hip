@@ -69,7 +69,6 @@ __device__ __forceinline__ void stream_level(const MakeSl& make_sl, const Persis
     if (threadIdx.x == 0) s.ok = ctl->err;
     __syncthreads();
     if (s.ok) return;
-    const auto sl = make_sl();
     const Board* cur = p.roots;
     const u32* cur_r = p.root_ids;
     u64 n = p.n_roots, off = 0;
@@ -88,7 +87,12 @@ __device__ __forceinline__ void stream_level(const MakeSl& make_sl, const Persis
     a.err = &ctl->err;
     a.out_count = &ctl->out_count[L];
     a.tile_parents = level_tile_parents(n, gridDim.x, TP);
+    // workgroups past the level's tile count leave at once (block 0 stays:
+    // it places the next level); the others drain the tile queue
+    const u64 ntiles = (n + a.tile_parents - 1) / a.tile_parents;
     if (MODE == kLeaf) {
+        if (blockIdx.x >= ntiles) return;
+        const auto sl = make_sl();
         a.out = nullptr;
         a.out_root = nullptr;
         a.out_cap = 0;
@@ -104,6 +108,8 @@ __device__ __forceinline__ void stream_level(const MakeSl& make_sl, const Persis
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->lvl_off[L + 1] = off;
+    if (blockIdx.x > 0 && blockIdx.x >= ntiles) return;
+    const auto sl = make_sl();
     a.out = reinterpret_cast<Board*>(p.arena + off);
     a.out_root = reinterpret_cast<u32*>(p.arena + off + bbytes);
     a.out_cap = nn;

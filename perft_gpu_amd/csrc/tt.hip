@@ -519,6 +519,7 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
     const auto sl = SlOf<B>::make(tabs);
     const u32 s_lo = (M == kLeaf) ? t->nseg_write : 0u, s_hi = (M == kWrite) ? t->nseg_write : t->nseg;
     const u32 first = t->tile_prefix[s_lo], total = t->tile_prefix[s_hi];
+    if (blockIdx.x >= total - first) return;  // more workgroups than tiles: the others drain the queue
     u32* const ctr = (M == kLeaf) ? &t->tile_ctr2 : &t->tile_ctr;
     u32 cur = 0xFFFFFFFFu;
     for (;;) {
@@ -566,6 +567,7 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
     const auto sl = SlOf<B>::make(tabs);
     const u32 nw = t->nseg_write, ns = t->nseg;
     const u32 wt = t->tile_prefix[nw], lt = t->tile_prefix[ns];  // write tiles [0, wt), leaf tiles [wt, lt)
+    if (blockIdx.x >= lt) return;  // more workgroups than tiles: the others drain the queues
     bool leaf = blockIdx.x >= t->mix_write_wgs;  // thread 0's queue
     bool moved = false;
     u32 cur = 0xFFFFFFFFu;
