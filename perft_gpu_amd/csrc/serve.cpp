@@ -61,18 +61,17 @@ pf_status Server::start() {
     max_region_ = job_bytes_ ? job_bytes_ : ((e_.arena_bytes / 4) & ~(kAl - 1));
     if (hipSetDevice(e_.device) != hipSuccess) return PF_EDEVICE;
     if (hipHostMalloc(&h_tab_, sizeof(StepTable), hipHostMallocDefault) != hipSuccess) return PF_ENOMEM;
-    if (hipHostMalloc(&h_acc_, sizeof(u64) * max_jobs_, hipHostMallocDefault) != hipSuccess) return PF_ENOMEM;
-    if (hipHostMalloc(&h_stats_, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
-        return PF_ENOMEM;
-    memset(h_stats_, 0, 4 * sizeof(unsigned long long));
     if (hipMalloc(&d_tab_, sizeof(StepTable)) != hipSuccess) return PF_ENOMEM;
+    // the slot accumulators and the table counters live in the step table's
+    // device-written part, which starts at zero
+    if (hipMemset(d_tab_, 0, sizeof(StepTable)) != hipSuccess) return PF_EDEVICE;
+    slot_acc_ = d_tab_->acc;
     const int n = max_jobs_;
     if (hipMalloc(&root_b_, sizeof(Board) * n) != hipSuccess || hipMalloc(&root_w_, 4 * n) != hipSuccess ||
         hipMalloc(&root_k1_, 8 * n) != hipSuccess || hipMalloc(&root_k2_, 8 * n) != hipSuccess ||
-        hipMalloc(&root_p_, 4 * n) != hipSuccess || hipMalloc(&slot_acc_, 8 * n) != hipSuccess)
+        hipMalloc(&root_p_, 4 * n) != hipSuccess)
         return PF_ENOMEM;
-    hipMemset(root_p_, 0, 4 * n);
-    if (e_.d_ttstats && hipMemset(e_.d_ttstats, 0, 8 * sizeof(unsigned long long)) != hipSuccess) return PF_EDEVICE;
+    if (hipMemset(root_p_, 0, 4 * n) != hipSuccess) return PF_EDEVICE;
     memset(h_tab_, 0, sizeof(StepTable));
     jobs_.assign((size_t)n, Job{});
     seg_tmp_.assign((size_t)kMaxSegs, TTArgs{});
@@ -103,17 +102,16 @@ pf_status Server::start() {
 pf_status Server::build_graph() {
     hipStream_t st = e_.stream;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return PF_EDEVICE;
-    hipError_t e = hipMemcpyAsync(d_tab_, h_tab_, sizeof(StepTable), hipMemcpyHostToDevice, st);
+    // the host's part of the table in, the device's part (outcomes, slot
+    // accumulators, table counters) out: one copy each way
+    constexpr size_t kHostPart = offsetof(StepTable, out);
+    hipError_t e = hipMemcpyAsync(d_tab_, h_tab_, kHostPart, hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = launch_step(e_.backend, e_.d_tab, e_.d_z, d_tab_, e_.d_tt, e_.tt_buckets ? e_.tt_mask_word() : 0, grid_,
                         up_grid_, st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(&h_tab_->out[0], &d_tab_->out[0], sizeof(h_tab_->out) + sizeof(h_tab_->rout),
-                           hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(h_acc_, slot_acc_, sizeof(u64) * max_jobs_, hipMemcpyDeviceToHost, st);
-    // the table's probe and hit counters (expand segments), for pf_get_stats
-    if (e == hipSuccess && e_.d_ttstats)
-        e = hipMemcpyAsync(h_stats_, e_.d_ttstats, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+        e = hipMemcpyAsync(reinterpret_cast<char*>(h_tab_) + kHostPart, reinterpret_cast<char*>(d_tab_) + kHostPart,
+                           sizeof(StepTable) - kHostPart, hipMemcpyDeviceToHost, st);
     // the capture is ended whatever happened inside it, so the stream leaves capture mode
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(st, &g);
@@ -162,13 +160,9 @@ void Server::stop() {
     hipFree(root_k1_);
     hipFree(root_k2_);
     hipFree(root_p_);
-    hipFree(slot_acc_);
     hipHostFree(h_tab_);
-    hipHostFree(h_acc_);
-    hipHostFree(h_stats_);
-    h_stats_ = nullptr;
     d_tab_ = nullptr; root_b_ = nullptr; root_w_ = nullptr; root_k1_ = nullptr; root_k2_ = nullptr;
-    root_p_ = nullptr; slot_acc_ = nullptr; h_tab_ = nullptr; h_acc_ = nullptr;
+    root_p_ = nullptr; slot_acc_ = nullptr; h_tab_ = nullptr;
 }
 
 // Predicted peak frontier bytes of a request: the written levels are plies
@@ -405,7 +399,7 @@ bool Server::plan(Job& j) {
     StepTable& t = *h_tab_;
     for (;;) {
         if (j.st.empty()) {
-            finish(j, h_acc_[j.slot], PF_OK);
+            finish(j, h_tab_->acc[j.slot], PF_OK);
             return false;
         }
         Frame& f = j.st.back();
@@ -429,7 +423,7 @@ bool Server::plan(Job& j) {
             a.tt_mask = e_.tt_buckets ? e_.tt_mask_word() : 0;
             a.ep_always = e_.exact_ep ? 0 : 1;
             a.child_depth = r - 1;
-            a.stats = e_.d_ttstats;
+            a.stats = d_tab_->stats;
             const u32 si = t.nseg;
             a.out_count = &d_tab_->out[si].out_count;
             a.err = &d_tab_->out[si].err;
@@ -687,7 +681,6 @@ void Server::run() {
         }
         t.up_round[kUpRounds] = t.nup;
         t.up_prefix[t.nup] = acc_u;
-        memset(t.out, 0, sizeof(SegOut) * t.nseg);
         const double t0 = now_s();
         const bool ok = hipGraphLaunch(exec_, e_.stream) == hipSuccess && hipStreamSynchronize(e_.stream) == hipSuccess;
         const double t1 = now_s();
@@ -697,8 +690,8 @@ void Server::run() {
             segs_ += t.nseg + t.nup + t.nroot;
             busy_ += t1 - t0;
             if (ok && e_.d_tt) {
-                probes_ = h_stats_[0];
-                hits_ = h_stats_[1];
+                probes_ = h_tab_->stats[0];
+                hits_ = h_tab_->stats[1];
                 for (u32 i = 0; i < t.nroot; i++) {
                     if (t.root[i].depth < kMinTTDepth) continue;
                     root_probes_++;
@@ -720,11 +713,11 @@ void Server::run() {
             const int pend = j.pending;
             j.pending = 0;
             if (pend == 1 || pend == 4) {  // a new root alone this step
-                if (j.depth == 1) finish(j, h_acc_[j.slot], PF_OK);
+                if (j.depth == 1) finish(j, h_tab_->acc[j.slot], PF_OK);
                 continue;
             }
             if (pend == 3) {  // upsweeps alone
-                if (after_chain(j, j.chain)) finish(j, h_acc_[j.slot], PF_OK);
+                if (after_chain(j, j.chain)) finish(j, h_tab_->acc[j.slot], PF_OK);
                 continue;
             }
             if (pend != 2) continue;
@@ -737,7 +730,7 @@ void Server::run() {
             Frame& f = j.st.back();
             if (f.state == ST_LEAF) {
                 j.st.pop_back();
-                if (j.st.empty() || after_chain(j, j.chain)) finish(j, h_acc_[j.slot], PF_OK);
+                if (j.st.empty() || after_chain(j, j.chain)) finish(j, h_tab_->acc[j.slot], PF_OK);
                 continue;
             }
             f.nx.ndup = o.ndup;

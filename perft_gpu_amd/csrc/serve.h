@@ -92,9 +92,7 @@ class Server {
     u64* root_k1_ = nullptr;
     u64* root_k2_ = nullptr;
     u32* root_p_ = nullptr;
-    unsigned long long* slot_acc_ = nullptr;
-    unsigned long long* h_acc_ = nullptr;
-    unsigned long long* h_stats_ = nullptr;  // the table counters, copied back by every step
+    unsigned long long* slot_acc_ = nullptr;  // d_tab_->acc
     u64 probes_ = 0, hits_ = 0, root_probes_ = 0, root_hits_ = 0;
     int grid_ = 0, up_grid_ = 0;
     std::vector<TTArgs> seg_tmp_;  // reordering buffer of a step's segments

@@ -639,13 +639,15 @@ __global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 ma
     }
 }
 
-// Start of a step: new requests (table probe at full depth, move count) and
-// empty duplicate tables for this step's write segments (a fill kernel: the
-// step is a captured graph, which records no memset).
+// Start of a step: the step's segment outcomes zeroed, new requests (table
+// probe at full depth, move count) and empty duplicate tables for this step's
+// write segments (a fill kernel: the step is a captured graph, which records
+// no memset).
 template <int B>
 __global__ __launch_bounds__(256) void k_pre_step(const SliderTables* tabs, StepTable* t, const uint4* tt, u64 mask) {
     const auto sl = SlOf<B>::make(tabs);
     const u32 gi = blockIdx.x * 256 + threadIdx.x;
+    if (gi < t->nseg) t->out[gi] = SegOut{};
     if (gi < t->nroot) {
         const RootSeg& r = t->root[gi];
         const Board b = r.b;
@@ -714,11 +716,12 @@ int step_grid(int backend, int n_cu) {
 }
 
 // One step: start (roots, fills), expand/leaf segments in one mixed launch,
-// kUpRounds upsweep rounds.  up_grid * 256 must cover kMaxRoots (the roots
-// are handled by the first threads of the start kernel).
+// kUpRounds upsweep rounds.  up_grid * 256 must cover kMaxRoots and kMaxSegs
+// (the roots and the outcomes are handled by the first threads of the start
+// kernel).
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
                        int grid, int up_grid, hipStream_t s) {
-    if (up_grid * 256 < kMaxRoots) return hipErrorInvalidValue;
+    if (up_grid * 256 < kMaxRoots || up_grid * 256 < kMaxSegs) return hipErrorInvalidValue;
     switch (backend) {
     case kKS:
         hipLaunchKernelGGL(k_pre_step<kKS>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask);

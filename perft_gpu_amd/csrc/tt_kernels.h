@@ -139,8 +139,12 @@ struct StepTable {
     TTArgs seg[kMaxSegs];
     UpSeg up[kMaxUps];
     RootSeg root[kMaxRoots];
-    SegOut out[kMaxSegs];
+    // device-written part, from out[] on: the host copies in everything
+    // before it and reads back everything from it, one copy each way
+    SegOut out[kMaxSegs];          // zeroed by the start kernel for [0, nseg)
     RootOut rout[kMaxRoots];
+    unsigned long long acc[kMaxRoots];  // serving: each job slot's root accumulator
+    unsigned long long stats[2];        // serving: table probes and hits of the write segments
 };
 
 // ---- device-planned table pass ----------------------------------------------
