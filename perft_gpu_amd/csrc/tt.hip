@@ -37,10 +37,11 @@ constexpr int kTTWaves = 4;
 // 2.189 -> 2.162 s against four waves without spills.
 constexpr int kTTLeafWaves = 5;
 // The serving step's mixed launch (write and leaf tiles at once) is built for
-// five too: 96 VGPRs, 96 B per lane of spills, five workgroups per CU.  On the
-// 6,018-request serving stream submitted at once: 7.60e12 leaves/s for the
-// write and leaf launches one after the other, 7.96e12 mixed at four waves,
-// 8.04-8.12e12 at five, 8.36-8.40e12 once a tile is taken with one atomic.
+// five too: 96 VGPRs, 96 B per lane of spills (none in the leaf loop), five
+// workgroups per CU.  On the 6,018-request serving stream submitted at once:
+// 7.60e12 leaves/s for the write and leaf launches one after the other,
+// 7.96e12 mixed at four waves, 8.04-8.12e12 at five, 8.36-8.40e12 once a tile
+// is taken with one atomic.
 constexpr int kTTMixedWaves = 5;
 
 // Adds each lane's v to its *p with one global atomic per run of consecutive
