@@ -14,13 +14,15 @@
 // stack, so that it can advance one GPU action at a time.
 //
 // One scheduler step = one replay of a captured graph:
-//   H2D step table -> start kernel (new requests: table probe at full depth,
-//   move count; empties the duplicate tables and block sums of this step's
-//   write segments) -> step kernel (the expand/leaf levels of every active
-//   job in one persistent launch over a shared tile queue; duplicates are
-//   listed as they are made) -> kUpRounds upsweep kernels (finished levels
-//   of every job, each round the levels just above the previous round's) ->
-//   D2H of per-segment outcomes and of the slot accumulators.
+//   H2D of the step table's host part -> start kernel (zeroes the segment
+//   outcomes; new requests: table probe at full depth, move count; empties
+//   the duplicate tables and block sums of this step's write segments) ->
+//   step kernel (the write and leaf levels of every active job in one
+//   persistent launch over two tile queues; duplicates are listed as they are
+//   made) -> up to kUpRounds upsweep kernels (finished levels of every job,
+//   each round the levels just above the previous round's; one graph per
+//   round count) -> D2H of the table's device part: per-segment outcomes,
+//   slot accumulators, table counters.
 // Requests that arrive while a step runs join the next step; their work
 // overlaps on the device with everyone else's.  All jobs share the lockless
 // transposition table.
@@ -99,7 +101,10 @@ pf_status Server::start() {
     return PF_OK;
 }
 
-pf_status Server::build_graph() {
+// One graph per number of upsweep rounds a step needs (0..kUpRounds): a step
+// replays the one with as many rounds as its upsweep chains fill, so a small
+// step does not pay for empty upsweep launches.
+pf_status Server::build_graph(int rounds) {
     hipStream_t st = e_.stream;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return PF_EDEVICE;
     // the host's part of the table in, the device's part (outcomes, slot
@@ -108,7 +113,7 @@ pf_status Server::build_graph() {
     hipError_t e = hipMemcpyAsync(d_tab_, h_tab_, kHostPart, hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = launch_step(e_.backend, e_.d_tab, e_.d_z, d_tab_, e_.d_tt, e_.tt_buckets ? e_.tt_mask_word() : 0, grid_,
-                        up_grid_, st);
+                        up_grid_, rounds, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(reinterpret_cast<char*>(h_tab_) + kHostPart, reinterpret_cast<char*>(d_tab_) + kHostPart,
                            sizeof(StepTable) - kHostPart, hipMemcpyDeviceToHost, st);
@@ -119,8 +124,8 @@ pf_status Server::build_graph() {
         if (g) hipGraphDestroy(g);
         return PF_EDEVICE;
     }
-    graph_ = g;
-    if (hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0) != hipSuccess) return PF_EDEVICE;
+    graph_[rounds] = g;
+    if (hipGraphInstantiate(&exec_[rounds], g, nullptr, nullptr, 0) != hipSuccess) return PF_EDEVICE;
     return PF_OK;
 }
 
@@ -150,10 +155,12 @@ void Server::stop() {
         std::lock_guard<std::mutex> g(mu_);
         running_ = false;
     }
-    if (exec_) hipGraphExecDestroy(exec_);
-    if (graph_) hipGraphDestroy(graph_);
-    exec_ = nullptr;
-    graph_ = nullptr;
+    for (int r = 0; r <= kUpRounds; r++) {
+        if (exec_[r]) hipGraphExecDestroy(exec_[r]);
+        if (graph_[r]) hipGraphDestroy(graph_[r]);
+        exec_[r] = nullptr;
+        graph_[r] = nullptr;
+    }
     hipFree(d_tab_);
     hipFree(root_b_);
     hipFree(root_w_);
@@ -544,7 +551,8 @@ bool Server::plan(Job& j) {
 
 void Server::run() {
     {
-        const pf_status gs = hipSetDevice(e_.device) == hipSuccess ? build_graph() : PF_EDEVICE;
+        pf_status gs = hipSetDevice(e_.device) == hipSuccess ? PF_OK : PF_EDEVICE;
+        for (int r = 0; r <= kUpRounds && gs == PF_OK; r++) gs = build_graph(r);
         std::lock_guard<std::mutex> g(mu_);
         graph_status_ = gs;
     }
@@ -682,7 +690,10 @@ void Server::run() {
         t.up_round[kUpRounds] = t.nup;
         t.up_prefix[t.nup] = acc_u;
         const double t0 = now_s();
-        const bool ok = hipGraphLaunch(exec_, e_.stream) == hipSuccess && hipStreamSynchronize(e_.stream) == hipSuccess;
+        int rounds = 0;  // rounds are filled in order: round r + 1 only after round r
+        while (rounds < kUpRounds && t.up_round[rounds + 1] > t.up_round[rounds]) rounds++;
+        const bool ok =
+            hipGraphLaunch(exec_[rounds], e_.stream) == hipSuccess && hipStreamSynchronize(e_.stream) == hipSuccess;
         const double t1 = now_s();
         {
             std::lock_guard<std::mutex> g(mu_);

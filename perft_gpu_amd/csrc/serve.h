@@ -64,7 +64,7 @@ class Server {
     void finish(Job& j, u64 nodes, int status);
     char* take(Job& j, u64 bytes);
     bool alloc_level(Job& j, u64 n, Engine::TLevel* lv);
-    pf_status build_graph();
+    pf_status build_graph(int rounds);
     u32 mix_write_wgs(u32 write_tiles, u32 leaf_tiles) const;
 
     Engine& e_;
@@ -96,8 +96,8 @@ class Server {
     u64 probes_ = 0, hits_ = 0, root_probes_ = 0, root_hits_ = 0;
     int grid_ = 0, up_grid_ = 0;
     std::vector<TTArgs> seg_tmp_;  // reordering buffer of a step's segments
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t exec_ = nullptr;
+    hipGraph_t graph_[kUpRounds + 1] = {};        // by upsweep rounds
+    hipGraphExec_t exec_[kUpRounds + 1] = {};
     u64 steps_ = 0, segs_ = 0, dups_ = 0;
     double busy_ = 0;
 };

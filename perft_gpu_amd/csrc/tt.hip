@@ -717,12 +717,13 @@ int step_grid(int backend, int n_cu) {
 }
 
 // One step: start (roots, fills), expand/leaf segments in one mixed launch,
-// kUpRounds upsweep rounds.  up_grid * 256 must cover kMaxRoots and kMaxSegs
-// (the roots and the outcomes are handled by the first threads of the start
-// kernel).
+// up_rounds (<= kUpRounds) upsweep rounds.  up_grid * 256 must cover
+// kMaxRoots and kMaxSegs (the roots and the outcomes are handled by the first
+// threads of the start kernel).
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
-                       int grid, int up_grid, hipStream_t s) {
+                       int grid, int up_grid, int up_rounds, hipStream_t s) {
     if (up_grid * 256 < kMaxRoots || up_grid * 256 < kMaxSegs) return hipErrorInvalidValue;
+    if (up_rounds < 0 || up_rounds > kUpRounds) return hipErrorInvalidValue;
     switch (backend) {
     case kKS:
         hipLaunchKernelGGL(k_pre_step<kKS>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask);
@@ -741,7 +742,7 @@ hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt,
         hipLaunchKernelGGL((k_step_mixed<kByte, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
         break;
     }
-    for (int r = 0; r < kUpRounds; r++) hipLaunchKernelGGL(k_up_step, dim3(up_grid), dim3(256), 0, s, t, tt, mask, r);
+    for (int r = 0; r < up_rounds; r++) hipLaunchKernelGGL(k_up_step, dim3(up_grid), dim3(256), 0, s, t, tt, mask, r);
     return hipGetLastError();
 }
 

@@ -205,7 +205,7 @@ hipError_t capture_plan_pass(int backend, const SliderTables* tabs, const ZTable
 int step_grid(int backend, int n_cu);       // workgroups of the serving step's mixed launch
 int step_max_bpc(int backend, int leaf);  // the planned pass's write (0) / leaf (1) step kernel
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
-                       int grid, int up_grid, hipStream_t s);
+                       int grid, int up_grid, int up_rounds, hipStream_t s);
 int expand_tt_max_bpc(int backend);   // most resident workgroups per CU of the TT write-level kernel
 int expand_tt_leaf_bpc(int backend);  // the same for the TT leaf-level kernel
 hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables* zt, const TTArgs& a, int grid,
