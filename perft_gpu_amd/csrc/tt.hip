@@ -616,8 +616,13 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
 // Upsweep round `round`: thread i of the round's concatenated segments (nodes
 // and duplicates) finds its segment by search.
 __global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 mask, int round) {
+    __shared__ u32 pre[kMaxUps + 1];
     const u32 s0 = t->up_round[round], s1 = t->up_round[round + 1];
     const u32 first = t->up_prefix[s0], last = t->up_prefix[s1];
+    if (first + blockIdx.x * 256 >= last) return;  // no item for this workgroup
+    // the round's segment boundaries, which every item searches, from LDS
+    for (u32 k = threadIdx.x; k <= s1 - s0; k += 256) pre[k] = t->up_prefix[s0 + k];
+    __syncthreads();
     unsigned long long stores = 0;
     // the loop is uniform per wave (wave_run_add needs every lane)
     for (u32 w = first + blockIdx.x * 256 + (threadIdx.x & ~63u); w < last; w += gridDim.x * 256) {
@@ -625,12 +630,12 @@ __global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 ma
         unsigned long long v = 0;
         unsigned long long* dst = nullptr;
         if (i < last) {
-            u32 lo = s0, hi = s1;  // largest segment with up_prefix[seg] <= i
+            u32 lo = 0, hi = s1 - s0;  // largest segment with up_prefix[s0 + lo] <= i
             while (hi - lo > 1) {
                 const u32 mid = (lo + hi) >> 1;
-                if (t->up_prefix[mid] <= i) lo = mid; else hi = mid;
+                if (pre[mid] <= i) lo = mid; else hi = mid;
             }
-            stores += upsweep_item(t->up[lo], i - t->up_prefix[lo], tt, mask, &dst, &v);
+            stores += upsweep_item(t->up[s0 + lo], i - pre[lo], tt, mask, &dst, &v);
         }
         wave_run_add(dst, v);
     }
