@@ -615,7 +615,7 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
 
 // Upsweep round `round`: thread i of the round's concatenated segments (nodes
 // and duplicates) finds its segment by search.
-__global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 mask, int round) {
+__global__ __launch_bounds__(256) void k_up_step(const StepTable* __restrict__ t, uint4* tt, u64 mask, int round) {
     __shared__ u32 pre[kMaxUps + 1];
     const u32 s0 = t->up_round[round], s1 = t->up_round[round + 1];
     const u32 first = t->up_prefix[s0], last = t->up_prefix[s1];
@@ -629,12 +629,20 @@ __global__ __launch_bounds__(256) void k_up_step(StepTable* t, uint4* tt, u64 ma
         const u32 i = w + (threadIdx.x & 63);
         unsigned long long v = 0;
         unsigned long long* dst = nullptr;
+        u32 lo = 0;
         if (i < last) {
-            u32 lo = 0, hi = s1 - s0;  // largest segment with up_prefix[s0 + lo] <= i
+            u32 hi = s1 - s0;  // largest segment with up_prefix[s0 + lo] <= i
             while (hi - lo > 1) {
                 const u32 mid = (lo + hi) >> 1;
                 if (pre[mid] <= i) lo = mid; else hi = mid;
             }
+        }
+        // most waves lie in one segment: its arguments then come through
+        // scalar loads, once per wave
+        const u32 lo0 = __builtin_amdgcn_readfirstlane(lo);
+        if (__all(i >= last || lo == lo0)) {
+            if (i < last) stores += upsweep_item(t->up[s0 + lo0], i - pre[lo0], tt, mask, &dst, &v);
+        } else if (i < last) {
             stores += upsweep_item(t->up[s0 + lo], i - pre[lo], tt, mask, &dst, &v);
         }
         wave_run_add(dst, v);
