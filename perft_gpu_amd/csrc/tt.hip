@@ -657,8 +657,11 @@ __global__ __launch_bounds__(256) void k_up_step(const StepTable* __restrict__ t
 // probe at full depth, move count) and empty duplicate tables for this step's
 // write segments (a fill kernel: the step is a captured graph, which records
 // no memset).
+// `segs` is t->seg, only read here: the segments' arguments come through
+// scalar loads.
 template <int B>
-__global__ __launch_bounds__(256) void k_pre_step(const SliderTables* tabs, StepTable* t, const uint4* tt, u64 mask) {
+__global__ __launch_bounds__(256) void k_pre_step(const SliderTables* tabs, StepTable* t,
+                                                  const TTArgs* __restrict__ segs, const uint4* tt, u64 mask) {
     const auto sl = SlOf<B>::make(tabs);
     const u32 gi = blockIdx.x * 256 + threadIdx.x;
     if (gi < t->nseg) t->out[gi] = SegOut{};
@@ -685,8 +688,9 @@ __global__ __launch_bounds__(256) void k_pre_step(const SliderTables* tabs, Step
         }
     }
     const u64 stride = (u64)gridDim.x * 256;
-    for (u32 si = 0; si < t->nseg; si++) {
-        const TTArgs& a = t->seg[si];
+    const u32 nw = t->nseg_write;  // leaf segments [nw, nseg) have no tables
+    for (u32 si = 0; si < nw; si++) {
+        const TTArgs& a = segs[si];
         if (a.out_bsum)
             for (u64 i = gi; i < a.bsum_n; i += stride) a.out_bsum[i] = 0;
         if (!a.dup) continue;
@@ -739,19 +743,19 @@ hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt,
     if (up_rounds < 0 || up_rounds > kUpRounds) return hipErrorInvalidValue;
     switch (backend) {
     case kKS:
-        hipLaunchKernelGGL(k_pre_step<kKS>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask);
+        hipLaunchKernelGGL(k_pre_step<kKS>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
         hipLaunchKernelGGL((k_step_mixed<kKS, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
         break;
     case kPlain:
-        hipLaunchKernelGGL(k_pre_step<kPlain>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask);
+        hipLaunchKernelGGL(k_pre_step<kPlain>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
         hipLaunchKernelGGL((k_step_mixed<kPlain, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
         break;
     case kFancy:
-        hipLaunchKernelGGL(k_pre_step<kFancy>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask);
+        hipLaunchKernelGGL(k_pre_step<kFancy>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
         hipLaunchKernelGGL((k_step_mixed<kFancy, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
         break;
     default:
-        hipLaunchKernelGGL(k_pre_step<kByte>, dim3(up_grid), dim3(256), 0, s, tabs, t, tt, mask);
+        hipLaunchKernelGGL(k_pre_step<kByte>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
         hipLaunchKernelGGL((k_step_mixed<kByte, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
         break;
     }
