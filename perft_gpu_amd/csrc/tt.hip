@@ -558,17 +558,21 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
 // which keep the VALU busy; a workgroup whose queue is empty moves to the
 // other one, and leaves when that is empty too.  The queues are the write
 // segments [0, nseg_write) with tile_ctr and the leaf segments [nseg_write,
-// nseg) with tile_ctr2; their bounds are read once, before the loop, so that
-// taking a tile is one atomic (small steps are latency-bound).
+// nseg) with tile_ctr2; their bounds are read once, before the loop, and the
+// tile prefix is copied to LDS, so that taking a tile is one atomic and a
+// search in LDS (small steps are latency-bound).
 template <int B, int W>
 __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void k_step_mixed(const SliderTables* tabs, const ZTables* zt, StepTable* t) {
     __shared__ TTLds s;
     __shared__ TTArgs sa;
     __shared__ u32 s_tile, s_seg;
+    __shared__ u32 pre[kMaxSegs + 1];  // the tile prefix, searched for every tile
     const auto sl = SlOf<B>::make(tabs);
     const u32 nw = t->nseg_write, ns = t->nseg;
     const u32 wt = t->tile_prefix[nw], lt = t->tile_prefix[ns];  // write tiles [0, wt), leaf tiles [wt, lt)
     if (blockIdx.x >= lt) return;  // more workgroups than tiles: the others drain the queues
+    for (u32 k = threadIdx.x; k <= ns; k += kTWG) pre[k] = t->tile_prefix[k];
+    __syncthreads();
     bool leaf = blockIdx.x >= t->mix_write_wgs;  // thread 0's queue
     bool moved = false;
     u32 cur = 0xFFFFFFFFu;
@@ -590,10 +594,10 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
             }
             u32 seg = 0;
             if (g != 0xFFFFFFFFu) {
-                u32 lo = leaf ? nw : 0u, hi = leaf ? ns : nw;  // largest seg with tile_prefix[seg] <= g
+                u32 lo = leaf ? nw : 0u, hi = leaf ? ns : nw;  // largest seg with pre[seg] <= g
                 while (hi - lo > 1) {
                     const u32 mid = (lo + hi) >> 1;
-                    if (t->tile_prefix[mid] <= g) lo = mid; else hi = mid;
+                    if (pre[mid] <= g) lo = mid; else hi = mid;
                 }
                 seg = lo;
             }
@@ -609,7 +613,7 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
         }
         __syncthreads();
         const TTArgs a = uniform_args(sa);  // the segment's own mode
-        tt_tile(sl, a, g - t->tile_prefix[seg], zt, s);
+        tt_tile(sl, a, g - pre[seg], zt, s);
     }
 }
 
