@@ -126,6 +126,9 @@ pf_status Engine::run_plan(const TLevel& lv, u64 off, u64 m, u64 n_next, int L) 
     h_plan->ply = L;
     h_plan->dedup = dedup ? 1 : 0;
     h_plan->ep_always = exact_ep ? 0 : 1;
+    // the fused launch runs on the leaf grid the pass was captured with
+    const int lg = std::min(tt_leaf_grid[backend], step_max_bpc(backend, 1) * n_cu);
+    h_plan->fuse_min_tiles = (u32)(kFuseTilesPerWG * lg);
     if (timing) PF_HIP(hipEventRecord(ev0, stream));
     PF_HIP(hipGraphLaunch(ex, stream));
     if (timing) PF_HIP(hipEventRecord(ev1, stream));

@@ -28,6 +28,9 @@ constexpr int kTWG = 256;
 // about 4,200 and spilled a few dozen into a second window almost every tile.
 constexpr int kTTP = kTTTileParents;
 constexpr int kTCAP = 4096; // LDS move-list window
+// Move-list windows a tile can need (a position has at most 218 legal moves).
+constexpr int kTTMaxWin = 8;
+static_assert(kTTP * 218 <= kTTMaxWin * kTCAP, "a tile's moves fit kTTMaxWin windows");
 // The write kernels wait on random table and duplicate-claim traffic most of
 // the time: ask for at least this many resident waves per SIMD (register
 // budget 512 / waves) so that more of that latency is hidden.
@@ -152,6 +155,9 @@ struct TTLds {
     u32 kcur, dcur;   // window-local cursors of the write pass
     u32 base, dbase;  // the window's ranges in the level and in the duplicate list
     u32 ok;
+    u32 nwin;                    // write tiles: windows so far, and each one's
+    u32 wr_base[kTTMaxWin];      // written children [wr_base, wr_base + wr_n)
+    u32 wr_n[kTTMaxWin];
 };
 
 struct LdsEmitT {
@@ -233,6 +239,9 @@ __device__ __forceinline__ void write_window(const SL& sl, const TTArgs& a, cons
         s.base = base;
         s.dbase = dbase;
         s.ok = ok;
+        s.wr_base[s.nwin] = base;  // nwin < kTTMaxWin: a tile has at most that many windows
+        s.wr_n[s.nwin] = ok ? nk : 0u;
+        s.nwin++;
         s.kcur = 0;
         s.dcur = 0;
         s.nkeep = 0;
@@ -321,6 +330,7 @@ __device__ void tt_tile(const SL& sl, const TTArgs& a, const u32 tile, const ZTa
             s.nhit = 0;
             s.nkeep = 0;
             s.ndup = 0;
+            s.nwin = 0;
         }
         const u32 p0 = tile * kTTP;
         const u32 np = min((u32)kTTP, a.n_in - p0);
@@ -520,6 +530,7 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
     const auto sl = SlOf<B>::make(tabs);
     const u32 s_lo = (M == kLeaf) ? t->nseg_write : 0u, s_hi = (M == kWrite) ? t->nseg_write : t->nseg;
     const u32 first = t->tile_prefix[s_lo], total = t->tile_prefix[s_hi];
+    if (M == kWrite && t->fuse_leaf) return;  // the fused launch runs this level
     if (blockIdx.x >= total - first) return;  // more workgroups than tiles: the others drain the queue
     u32* const ctr = (M == kLeaf) ? &t->tile_ctr2 : &t->tile_ctr;
     u32 cur = 0xFFFFFFFFu;
@@ -549,6 +560,57 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
         TTArgs a = uniform_args(sa);
         a.mode = M;
         tt_tile(sl, a, g - t->tile_prefix[seg], zt, s);
+    }
+}
+
+// The planned pass's last write level and its leaf pass in one launch (its
+// segment 0, a write level whose children have remaining depth 2), when the
+// level's planner chose it (StepTable::fuse_leaf: enough tiles for every
+// workgroup; otherwise this launch leaves at once and the write and leaf
+// launches run the level).  A workgroup writes one tile's children and then
+// runs the leaf pass over the children it kept, at once, while they are still
+// in the caches.  Workgroups are in different phases at any time, so the
+// write tiles' waits on table and duplicate claims overlap other workgroups'
+// leaf VALU work, where a write launch followed by a leaf launch ran them one
+// after the other: C5's first 2048 positions at perft 6 0.934 -> 0.865 s, C3
+// perft 8 0.080 -> 0.076 s.  The kept children of a tile are the windows'
+// reserved ranges (TTLds::wr_base/wr_n), whose accumulators the write just
+// zeroed; the leaf tiles add perft(child, 2) to them.
+template <int B, int W>
+__global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void k_step_write_leaf(const SliderTables* tabs, const ZTables* zt, StepTable* t) {
+    __shared__ TTLds s;
+    __shared__ TTArgs sa;
+    __shared__ u32 s_tile;
+    const auto sl = SlOf<B>::make(tabs);
+    const u32 total = t->tile_prefix[1];
+    if (!t->fuse_leaf) return;  // the level was written by the write launch; the leaf launch follows
+    if (blockIdx.x >= total) return;  // more workgroups than tiles: the others drain the queue
+    if (threadIdx.x == 0) sa = t->seg[0];
+    __syncthreads();
+    TTArgs a = uniform_args(sa);
+    a.mode = kWrite;
+    TTArgs l = {};  // the leaf tiles' arguments: parents are the written children
+    l.mode = kLeaf;
+    l.child_depth = a.child_depth - 1;
+    l.err = a.err;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&t->tile_ctr, 1u);
+        __syncthreads();
+        const u32 g = s_tile;
+        if (g >= total) break;
+        tt_tile(sl, a, g, zt, s);  // ends with a barrier: s.nwin and the ranges are final
+        const u32 nwin = s.nwin;
+        __syncthreads();  // every wave has nwin before a leaf tile reuses the tile state
+        for (u32 w = 0; w < nwin; w++) {
+            const u32 rb = s.wr_base[w], rn = s.wr_n[w];
+            for (u32 c0 = 0; c0 < rn; c0 += kTTP) {
+                l.in = a.out + rb + c0;
+                l.in_root = a.out_root + rb + c0;
+                l.in_acc = a.out_acc + rb + c0;
+                l.n_in = min((u32)kTTP, rn - c0);
+                tt_tile(sl, l, 0, zt, s);
+            }
+        }
     }
 }
 
@@ -796,6 +858,7 @@ __device__ __forceinline__ void plan_empty(StepTable* t) {
     t->tile_ctr2 = 0;
     t->tile_prefix[0] = 0;
     t->tile_prefix[1] = 0;
+    t->fuse_leaf = 0;
     for (int r = 0; r <= kUpRounds; r++) t->up_round[r] = 0;
     t->up_prefix[0] = 0;
     t->out[0] = SegOut{};
@@ -893,9 +956,15 @@ __global__ __launch_bounds__(256) void k_plan_write(TTPlanCtl* c, StepTable* t, 
         a.dup_slot = nx.dslot;
     }
     t->seg[0] = a;
-    t->tile_prefix[1] = (u32)((cur.n + kTTP - 1) / kTTP);
+    const u32 tiles = (u32)((cur.n + kTTP - 1) / kTTP);
+    t->tile_prefix[1] = tiles;
     t->nseg = 1;
     t->nseg_write = 1;
+    // the last write level (children of remaining depth 2) fuses its leaf pass
+    // when every workgroup gets many tiles: a fused tile (its write, then the
+    // leaf tiles of its kept children) is long, and a few of them would leave
+    // most workgroups idle
+    t->fuse_leaf = (a.child_depth == 2 && tiles >= c->p.fuse_min_tiles) ? 1u : 0u;
 }
 
 // Empties level L + 1's duplicate table and, at L = 0, the pass's accumulators.
@@ -908,15 +977,19 @@ __global__ __launch_bounds__(256) void k_plan_clear(TTPlanCtl* c, int L) {
         for (u64 k = i0; k < c->lv[0].n; k += stride) c->lv[0].acc[k] = 0;
 }
 
-// Before the leaf pass over level L (remaining depth 2).
+// After the last write level (level L, remaining depth 2): records it and,
+// unless the write launch ran the leaf pass fused (StepTable::fuse_leaf),
+// plans the leaf pass over it.
 __global__ __launch_bounds__(64) void k_plan_leaf(TTPlanCtl* c, StepTable* t, int L,
                                                   unsigned long long* ttstats) {
     if (threadIdx.x != 0) return;
+    const bool fused = t->fuse_leaf != 0;
     if (!c->st.err) plan_record(c, t, L);
     plan_empty(t);
     if (c->st.err) return;
     const TTLevelRec& cur = c->lv[L];
     c->st.made += cur.next;
+    if (fused) return;  // the write launch ran the leaf pass: the leaf launch stays empty
     TTArgs a = {};
     a.in = cur.b;
     a.in_root = cur.root;
@@ -976,8 +1049,15 @@ static void plan_steps(const SliderTables* tabs, const ZTables* zt, TTPlanCtl* c
     for (int L = 0; L <= r - 3; L++) {
         hipLaunchKernelGGL(k_plan_write, dim3(1), dim3(256), 0, s, ctl, tab, L, tt, mask, ttstats);
         hipLaunchKernelGGL(k_plan_clear, dim3(up_grid), dim3(256), 0, s, ctl, L);
+        // the write launch; at the last write level, when its planner chose to
+        // fuse the leaf pass (StepTable::fuse_leaf), it leaves at once and the
+        // fused launch runs the level instead
         hipLaunchKernelGGL((k_step<B, kTTWaves, kWrite>), dim3(write_grid), dim3(kTWG), 0, s, tabs, zt, tab);
+        if (L == r - 3)
+            hipLaunchKernelGGL((k_step_write_leaf<B, kTTMixedWaves>), dim3(leaf_grid), dim3(kTWG), 0, s, tabs, zt,
+                               tab);
     }
+    // records the leaf level; the leaf launch is empty when the write fused it
     hipLaunchKernelGGL(k_plan_leaf, dim3(1), dim3(64), 0, s, ctl, tab, r - 2, ttstats);
     hipLaunchKernelGGL((k_step<B, kTTLeafWaves, kLeaf>), dim3(leaf_grid), dim3(kTWG), 0, s, tabs, zt, tab);
     for (int L = r - 3; L >= 0; L--) {

@@ -131,7 +131,10 @@ struct StepTable {
     u32 nseg_write, tile_ctr2;
     // the serving step's mixed launch: workgroups [0, mix_write_wgs) start on
     // the write tiles, the others on the leaf tiles
-    u32 mix_write_wgs, pad0;
+    u32 mix_write_wgs;
+    // planned pass: the last write level runs its leaf pass fused (set by its
+    // planner when the level has enough tiles for every workgroup)
+    u32 fuse_leaf;
     unsigned long long* stores;    // table stores of the upsweeps are counted here (null: not counted)
     u32 tile_prefix[kMaxSegs + 1];
     u32 up_prefix[kMaxUps + 1];    // item prefix over all upsweep segments
@@ -153,12 +156,16 @@ struct StepTable {
 // the device by a one-block planner from the counts the previous launch left
 // there, so no level returns to the host (the reference's per-level launch
 // and device sync, perft_bb.h:1071-1135).  Per write level: plan, clear the
-// level's duplicate table, expand; then the leaf pass; then one upsweep per
-// level on the way back up; then the commit.  The pass accumulates the
+// level's duplicate table, expand; the last write level runs fused with the
+// leaf pass over its children; then one upsweep per level on the way back up;
+// then the commit.  The pass accumulates the
 // parents' counts in an array of its own and adds them to the caller's only
 // at the commit, and only if no level overflowed: an overflowed pass leaves
 // the caller's accumulators, the arena below it and the table unchanged.
 constexpr int kPlanMaxLevels = 18;
+// The last write level fuses its leaf pass when it has at least this many
+// tiles of parents per workgroup of the launch.
+constexpr int kFuseTilesPerWG = 16;
 struct TTLevelRec {
     Board* b;
     u32* root;
@@ -182,6 +189,7 @@ struct TTPlanParams {  // the host's part, copied in by the graph's first node
     int ply;           // ply of the parents (duplicate tables from kDupMinPly)
     int dedup;
     int ep_always;     // TTArgs::ep_always of the pass's write levels
+    u32 fuse_min_tiles;  // the last write level fuses its leaf pass from this many tiles of parents
 };
 struct TTPlanStatus {  // copied back by the graph's last node
     u64 peak;          // arena bytes the pass used
