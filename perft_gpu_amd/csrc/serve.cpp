@@ -46,7 +46,13 @@ static constexpr u64 kAl = 256;
 // frame states: START plans a write or leaf level; WRITE / LEAF / UP are in
 // flight; WAITCHILD parks a parent under its child frame; CHILD (child done)
 // plans the upsweep; UPDONE frees the children and moves on.
-enum { ST_START = 0, ST_WRITE, ST_LEAF, ST_WAITCHILD, ST_CHILD, ST_UP, ST_UPDONE };
+enum { ST_START = 0, ST_WRITE, ST_LEAF, ST_WAITCHILD, ST_CHILD, ST_UP, ST_UPDONE, ST_WRITE_LATE };
+// A step may hold late leaf passes (the write of a level of remaining depth 2,
+// its leaf pass and the upsweeps above it, all in one step) when nothing waits
+// for admission and at most this many jobs are active: small steps, whose
+// latency it shortens by a step per job.  Busier steps overlap the write and
+// the leaf pass of different jobs in the mixed launch instead.
+static constexpr size_t kLateMaxJobs = 16;
 
 Server::Server(Engine& e, int max_jobs, u64 job_bytes) : e_(e), max_jobs_(max_jobs), job_bytes_(job_bytes) {}
 
@@ -77,6 +83,7 @@ pf_status Server::start() {
     memset(h_tab_, 0, sizeof(StepTable));
     jobs_.assign((size_t)n, Job{});
     seg_tmp_.assign((size_t)kMaxSegs, TTArgs{});
+    pos_of_.assign((size_t)kMaxSegs, 0u);
     free_slots_.clear();
     for (int i = n - 1; i >= 0; i--) free_slots_.push_back(i);
     grid_ = step_grid(e_.backend, e_.n_cu);
@@ -352,8 +359,10 @@ bool Server::chunk(Job& j, Frame& f, u64 budget, u64* cap) {
 // Frames st[k], st[k-1], ... have complete child levels: schedule their
 // upsweeps in rounds 0, 1, ... of this step, up to a frame that still has
 // chunks to run (its next chunk reuses the memory being upswept, so it waits
-// for the next step).  Returns the number scheduled.
-int Server::chain_up(Job& j, int k) {
+// for the next step).  dev_seg >= 0: st[k]'s child level is written in this
+// step by segment dev_seg, whose outcome holds its counts.  Returns the
+// number scheduled.
+int Server::chain_up(Job& j, int k, int dev_seg) {
     int n = 0;
     for (; k >= 0 && n < kUpRounds; k--) {
         if (nup_planned_ >= (u32)kMaxUps) break;
@@ -370,6 +379,10 @@ int Server::chain_up(Job& j, int k) {
         u.n = (u32)p.written;
         u.ndup = (u32)p.nx.ndup;
         u.depth = j.depth - p.L - 1;
+        if (dev_seg >= 0) {  // st[k]'s child level is written this step: every count of the chain waits on it
+            u.dev = &d_tab_->out[dev_seg];
+            u.dev_counts = n == 0 ? 1u : 0u;
+        }
         up_[(size_t)n].push_back(u);
         nup_planned_++;
         n++;
@@ -508,6 +521,13 @@ bool Server::plan(Job& j) {
                 }
                 f.cap = cap;
                 a.mode = kWrite;
+                // children of remaining depth 2 in a small step: their leaf pass
+                // and the upsweeps above them run in this step too
+                if (r == 3 && late_ok_ && t.nlate < (u32)kMaxLate && nup_planned_ < (u32)kMaxUps) {
+                    late_src_[t.nlate++] = si;
+                    f.state = ST_WRITE_LATE;
+                    j.chain = chain_up(j, (int)j.st.size() - 1, (int)si);
+                }
                 a.out = f.nx.b;
                 a.out_root = f.nx.root;
                 a.out_k1 = f.nx.k1;
@@ -518,7 +538,7 @@ bool Server::plan(Job& j) {
                 a.out_bsum = f.nx.bsum;
                 a.bsum_n = f.nx.bsum ? (u32)((cap + kBsumBlock - 1) >> kBsumShift) : 0u;
                 a.total = &d_tab_->out[si].total;
-                f.state = ST_WRITE;
+                if (f.state != ST_WRITE_LATE) f.state = ST_WRITE;
             }
             t.seg[si] = a;
             t.nseg++;
@@ -600,6 +620,9 @@ void Server::run() {
         StepTable& t = *h_tab_;
         t.nseg = t.nup = t.nroot = 0;
         t.tile_ctr = 0;
+        t.nlate = 0;
+        t.tile_ctr3 = 0;
+        late_ok_ = queue_idle_ && active.size() + admitted.size() <= kLateMaxJobs;
         for (auto& v : up_) v.clear();
         nup_planned_ = 0;
         // running jobs first, then the new ones: each new root is probed and
@@ -661,12 +684,19 @@ void Server::run() {
         {
             u32 nw = 0;
             for (u32 i = 0; i < t.nseg; i++)
-                if (t.seg[i].mode == kWrite) seg_tmp_[nw++] = t.seg[i];
+                if (t.seg[i].mode == kWrite) {
+                    pos_of_[i] = nw;
+                    seg_tmp_[nw++] = t.seg[i];
+                }
             t.nseg_write = nw;
             for (u32 i = 0; i < t.nseg; i++)
-                if (t.seg[i].mode != kWrite) seg_tmp_[nw++] = t.seg[i];
+                if (t.seg[i].mode != kWrite) {
+                    pos_of_[i] = nw;
+                    seg_tmp_[nw++] = t.seg[i];
+                }
             std::copy(seg_tmp_.begin(), seg_tmp_.begin() + t.nseg, t.seg);
             t.tile_ctr2 = 0;
+            for (u32 k = 0; k < t.nlate; k++) t.late_pos[k] = pos_of_[late_src_[k]];
         }
         // tile prefix over expand segments
         u32 acc_t = 0;
@@ -739,6 +769,10 @@ void Server::run() {
             }
             step_dups += o.ndup;
             Frame& f = j.st.back();
+            if (f.state == ST_WRITE_LATE) {  // written, counted and upswept in this step
+                if (after_chain(j, j.chain)) finish(j, h_tab_->acc[j.slot], PF_OK);
+                continue;
+            }
             if (f.state == ST_LEAF) {
                 j.st.pop_back();
                 if (j.st.empty() || after_chain(j, j.chain)) finish(j, h_tab_->acc[j.slot], PF_OK);

@@ -58,7 +58,7 @@ class Server {
     void region_free(u64 off, u64 bytes);
     bool grow_region(Job& j, u64 extra);
     bool plan(Job& j);
-    int chain_up(Job& j, int k);
+    int chain_up(Job& j, int k, int dev_seg = -1);
     bool chunk(Job& j, Frame& f, u64 budget, u64* cap);
     bool after_chain(Job& j, int n);
     void finish(Job& j, u64 nodes, int status);
@@ -96,6 +96,9 @@ class Server {
     u64 probes_ = 0, hits_ = 0, root_probes_ = 0, root_hits_ = 0;
     int grid_ = 0, up_grid_ = 0;
     std::vector<TTArgs> seg_tmp_;  // reordering buffer of a step's segments
+    std::vector<u32> pos_of_;      // a segment's position after the reordering
+    bool late_ok_ = false;         // this step may hold late leaf passes
+    u32 late_src_[kMaxLate] = {};  // their write segments (indices before the reordering)
     hipGraph_t graph_[kUpRounds + 1] = {};        // by upsweep rounds
     hipGraphExec_t exec_[kUpRounds + 1] = {};
     u64 steps_ = 0, segs_ = 0, dups_ = 0;

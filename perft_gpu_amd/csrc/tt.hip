@@ -463,14 +463,28 @@ __global__ __launch_bounds__(256) void k_roots_tt(const SliderTables* tabs, Boar
     atomicAdd(total, (unsigned long long)c);
 }
 
+// Nodes of an upsweep segment, and all its items (nodes, then duplicates):
+// from the host, or from the outcome of a write that ran in the same step;
+// none when that write overflowed, so that no count short of it is added up
+// or stored in the table.
+__device__ __forceinline__ u32 up_nodes(const UpSeg& u) {
+    if (u.dev && u.dev->err) return 0;  // the write below it overflowed: nothing goes up or into the table
+    return u.dev_counts ? u.dev->out_count : u.n;
+}
+__device__ __forceinline__ u32 up_items(const UpSeg& u) {
+    if (u.dev && u.dev->err) return 0;
+    return u.dev_counts ? u.dev->out_count + u.dev->ndup : u.n + u.ndup;
+}
+
 // Item k of a finished level's upsweep (calcPerftNFromPerftNminus1_hash128b,
-// perft_bb.h:854-961): k < n is node k, which adds its exact count to its
-// parent and is stored in the table; n <= k < n + ndup is duplicate k - n,
-// which adds the count of the node that claimed its slot.  The addition is
-// returned in (*dst, *v) for wave_run_add.  Returns 1 for a store.
-__device__ __forceinline__ u32 upsweep_item(const UpSeg& u, u32 k, uint4* tt, u64 mask, unsigned long long** dst,
-                                            unsigned long long* v) {
-    if (k < u.n) {
+// perft_bb.h:854-961) whose n nodes come first: k < n is node k, which adds
+// its exact count to its parent and is stored in the table; n <= k < n + ndup
+// is duplicate k - n, which adds the count of the node that claimed its slot.
+// The addition is returned in (*dst, *v) for wave_run_add.  Returns 1 for a
+// store.
+__device__ __forceinline__ u32 upsweep_item(const UpSeg& u, u32 n, u32 k, uint4* tt, u64 mask,
+                                            unsigned long long** dst, unsigned long long* v) {
+    if (k < n) {
         *v = u.acc[k];
         *dst = u.parent_acc ? &u.parent_acc[u.parent[k]] : nullptr;
         if (tt && u.depth >= kMinTTDepth) {
@@ -479,7 +493,7 @@ __device__ __forceinline__ u32 upsweep_item(const UpSeg& u, u32 k, uint4* tt, u6
         }
         return 0;
     }
-    const u32 d = k - u.n;
+    const u32 d = k - n;
     if (u.parent_acc) {
         const u32 c = (u32)u.dup[u.dup_slot[d]].val;
         *v = u.acc[c];
@@ -492,7 +506,7 @@ __global__ __launch_bounds__(256) void k_upsweep(UpSeg u, uint4* tt, u64 mask, u
     const u32 i = blockIdx.x * 256 + threadIdx.x;
     unsigned long long st = 0, v = 0;
     unsigned long long* dst = nullptr;
-    if (i < u.n + u.ndup) st = upsweep_item(u, i, tt, mask, &dst, &v);
+    if (i < u.n + u.ndup) st = upsweep_item(u, u.n, i, tt, mask, &dst, &v);
     wave_run_add(dst, v);
     if (stores) {
         st = wave_sum64(st);
@@ -679,25 +693,100 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
     }
 }
 
-// Upsweep round `round`: thread i of the round's concatenated segments (nodes
-// and duplicates) finds its segment by search.
+// Serving: the leaf passes over levels written by this step's write launch
+// (StepTable::late_pos), so that a job's level of remaining depth 2 is
+// written, counted and upswept in one step.  Each pass's parents are its write
+// segment's children; their number is that write's outcome (none if it
+// overflowed), so the tile prefix is built here, by the first wave.
+template <int B, int W>
+__global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void k_step_late(const SliderTables* tabs, const ZTables* zt, StepTable* t) {
+    __shared__ TTLds s;
+    __shared__ u32 pre[kMaxLate + 1];
+    __shared__ u32 s_tile, s_seg;
+    const u32 nl = t->nlate;
+    if (!nl) return;
+    const auto sl = SlOf<B>::make(tabs);
+    if (threadIdx.x < 64) {
+        u32 tiles = 0;
+        if (threadIdx.x < nl) {
+            const TTArgs& w = t->seg[t->late_pos[threadIdx.x]];
+            const u32 n = *w.err ? 0u : *w.out_count;
+            tiles = (n + kTTP - 1) / kTTP;
+        }
+        const u32 incl = wave_incl_scan(tiles);
+        if (threadIdx.x < nl) pre[threadIdx.x + 1] = incl;
+        if (threadIdx.x == 0) pre[0] = 0;
+    }
+    __syncthreads();
+    const u32 total = pre[nl];
+    if (blockIdx.x >= total) return;  // more workgroups than tiles: the others drain the queue
+    u32 cur = 0xFFFFFFFFu;
+    TTArgs l = {};
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const u32 g = atomicAdd(&t->tile_ctr3, 1u);
+            u32 lo = 0, hi = nl;  // largest k with pre[k] <= g
+            if (g < total)
+                while (hi - lo > 1) {
+                    const u32 mid = (lo + hi) >> 1;
+                    if (pre[mid] <= g) lo = mid; else hi = mid;
+                }
+            s_tile = g;
+            s_seg = lo;
+        }
+        __syncthreads();
+        const u32 g = s_tile, k = s_seg;
+        if (g >= total) break;
+        if (k != cur) {  // the write's children as parents (uniform: from LDS and the table)
+            const TTArgs& w = t->seg[t->late_pos[k]];
+            l.in = w.out;
+            l.in_root = w.out_root;
+            l.in_acc = w.out_acc;
+            l.n_in = *w.out_count;
+            l.mode = kLeaf;
+            l.child_depth = w.child_depth - 1;
+            l.err = w.err;
+            cur = k;
+        }
+        tt_tile(sl, l, g - pre[k], zt, s);
+    }
+}
+
+// Upsweep round `round`: item i of the round's concatenated segments (nodes
+// and duplicates) finds its segment by search.  Every workgroup first builds
+// the round's item prefix in LDS, as a segment written in this same step has
+// its counts only on the device.
 __global__ __launch_bounds__(256) void k_up_step(const StepTable* __restrict__ t, uint4* tt, u64 mask, int round) {
     __shared__ u32 pre[kMaxUps + 1];
-    const u32 s0 = t->up_round[round], s1 = t->up_round[round + 1];
-    const u32 first = t->up_prefix[s0], last = t->up_prefix[s1];
-    if (first + blockIdx.x * 256 >= last) return;  // no item for this workgroup
-    // the round's segment boundaries, which every item searches, from LDS
-    for (u32 k = threadIdx.x; k <= s1 - s0; k += 256) pre[k] = t->up_prefix[s0 + k];
+    __shared__ u32 wtot[4];
+    const u32 s0 = t->up_round[round], s1 = t->up_round[round + 1], ns = s1 - s0;
+    // pre[k] = items of segments [0, k): thread x sums segments 2x and 2x + 1
+    const u32 k0 = 2 * threadIdx.x, k1 = k0 + 1;
+    const u32 c0 = k0 < ns ? up_items(t->up[s0 + k0]) : 0u, c1 = k1 < ns ? up_items(t->up[s0 + k1]) : 0u;
+    const u32 incl = wave_incl_scan(c0 + c1);
+    if ((threadIdx.x & 63) == 63) wtot[threadIdx.x >> 6] = incl;
     __syncthreads();
+    u32 woff = 0, last = 0;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        woff += (w < (int)(threadIdx.x >> 6)) ? wtot[w] : 0u;
+        last += wtot[w];
+    }
+    const u32 excl = woff + incl - c0 - c1;
+    if (k0 < ns) pre[k0] = excl;
+    if (k1 < ns) pre[k1] = excl + c0;
+    if (threadIdx.x == 0) pre[ns] = last;
+    __syncthreads();
+    if (blockIdx.x * 256 >= last) return;  // no item for this workgroup
     unsigned long long stores = 0;
     // the loop is uniform per wave (wave_run_add needs every lane)
-    for (u32 w = first + blockIdx.x * 256 + (threadIdx.x & ~63u); w < last; w += gridDim.x * 256) {
+    for (u32 w = blockIdx.x * 256 + (threadIdx.x & ~63u); w < last; w += gridDim.x * 256) {
         const u32 i = w + (threadIdx.x & 63);
         unsigned long long v = 0;
         unsigned long long* dst = nullptr;
         u32 lo = 0;
         if (i < last) {
-            u32 hi = s1 - s0;  // largest segment with up_prefix[s0 + lo] <= i
+            u32 hi = ns;  // largest segment with pre[lo] <= i
             while (hi - lo > 1) {
                 const u32 mid = (lo + hi) >> 1;
                 if (pre[mid] <= i) lo = mid; else hi = mid;
@@ -707,9 +796,13 @@ __global__ __launch_bounds__(256) void k_up_step(const StepTable* __restrict__ t
         // scalar loads, once per wave
         const u32 lo0 = __builtin_amdgcn_readfirstlane(lo);
         if (__all(i >= last || lo == lo0)) {
-            if (i < last) stores += upsweep_item(t->up[s0 + lo0], i - pre[lo0], tt, mask, &dst, &v);
+            if (i < last) {
+                const UpSeg& u = t->up[s0 + lo0];
+                stores += upsweep_item(u, up_nodes(u), i - pre[lo0], tt, mask, &dst, &v);
+            }
         } else if (i < last) {
-            stores += upsweep_item(t->up[s0 + lo], i - pre[lo], tt, mask, &dst, &v);
+            const UpSeg& u = t->up[s0 + lo];
+            stores += upsweep_item(u, up_nodes(u), i - pre[lo], tt, mask, &dst, &v);
         }
         wave_run_add(dst, v);
     }
@@ -800,7 +893,8 @@ int step_grid(int backend, int n_cu) {
 }
 
 // One step: start (roots, fills), expand/leaf segments in one mixed launch,
-// up_rounds (<= kUpRounds) upsweep rounds.  up_grid * 256 must cover
+// the late leaf passes (over levels the mixed launch wrote), up_rounds
+// (<= kUpRounds) upsweep rounds.  up_grid * 256 must cover
 // kMaxRoots and kMaxSegs (the roots and the outcomes are handled by the first
 // threads of the start kernel).
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
@@ -811,18 +905,22 @@ hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt,
     case kKS:
         hipLaunchKernelGGL(k_pre_step<kKS>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
         hipLaunchKernelGGL((k_step_mixed<kKS, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+        hipLaunchKernelGGL((k_step_late<kKS, kTTLeafWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
         break;
     case kPlain:
         hipLaunchKernelGGL(k_pre_step<kPlain>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
         hipLaunchKernelGGL((k_step_mixed<kPlain, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+        hipLaunchKernelGGL((k_step_late<kPlain, kTTLeafWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
         break;
     case kFancy:
         hipLaunchKernelGGL(k_pre_step<kFancy>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
         hipLaunchKernelGGL((k_step_mixed<kFancy, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+        hipLaunchKernelGGL((k_step_late<kFancy, kTTLeafWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
         break;
     default:
         hipLaunchKernelGGL(k_pre_step<kByte>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
         hipLaunchKernelGGL((k_step_mixed<kByte, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+        hipLaunchKernelGGL((k_step_late<kByte, kTTLeafWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
         break;
     }
     for (int r = 0; r < up_rounds; r++) hipLaunchKernelGGL(k_up_step, dim3(up_grid), dim3(256), 0, s, t, tt, mask, r);

@@ -84,7 +84,16 @@ struct TTArgs {
 constexpr int kMaxSegs = 512;
 constexpr int kMaxUps = 512;
 constexpr int kMaxRoots = 256;
+// Serving: leaf passes per step over levels written in the same step.
+constexpr int kMaxLate = 16;
 
+struct SegOut {
+    u32 out_count;
+    u32 err;
+    unsigned long long total;
+    u32 ndup;  // children found to duplicate a written sibling of their level
+    u32 pad;
+};
 struct UpSeg {
     const u32* parent;
     const u64* k1;
@@ -96,6 +105,11 @@ struct UpSeg {
     const u32* dup_slot;
     u32 n, ndup;                 // nodes, then duplicates: n + ndup items
     int depth;
+    // serving, upsweeps that follow a write in the same step (a late leaf
+    // pass's chain): that write's outcome.  No items if it overflowed; with
+    // dev_counts, the level is that write's and its n and ndup are there.
+    const SegOut* dev;
+    u32 dev_counts;
 };
 struct RootSeg {  // payload travels in the table; the kernel scatters it
     Board b;
@@ -107,13 +121,6 @@ struct RootSeg {  // payload travels in the table; the kernel scatters it
     u64* dst_k2;
     u32* dst_w;
     unsigned long long* acc;
-};
-struct SegOut {
-    u32 out_count;
-    u32 err;
-    unsigned long long total;
-    u32 ndup;  // children found to duplicate a written sibling of their level
-    u32 pad;
 };
 struct RootOut {
     u32 hit;
@@ -135,9 +142,15 @@ struct StepTable {
     // planned pass: the last write level runs its leaf pass fused (set by its
     // planner when the level has enough tiles for every workgroup)
     u32 fuse_leaf;
+    // serving: leaf passes over levels written in this same step, run after
+    // the write launch ("late"): late_pos[k] is the write segment's position in
+    // seg[]; their tiles come from tile_ctr3
+    u32 nlate, tile_ctr3;
+    u32 late_pos[kMaxLate];
     unsigned long long* stores;    // table stores of the upsweeps are counted here (null: not counted)
     u32 tile_prefix[kMaxSegs + 1];
-    u32 up_prefix[kMaxUps + 1];    // item prefix over all upsweep segments
+    u32 up_prefix[kMaxUps + 1];    // item prefix over all upsweep segments (the upsweep kernel recomputes
+                                   // it per round, as some counts are only known on the device)
     u32 up_round[kUpRounds + 1];   // segments [up_round[r], up_round[r+1]) belong to round r
     TTArgs seg[kMaxSegs];
     UpSeg up[kMaxUps];
