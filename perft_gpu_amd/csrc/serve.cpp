@@ -68,7 +68,9 @@ pf_status Server::start() {
     free_mem_[0] = e_.arena_bytes & ~(kAl - 1);
     max_region_ = job_bytes_ ? job_bytes_ : ((e_.arena_bytes / 4) & ~(kAl - 1));
     if (hipSetDevice(e_.device) != hipSuccess) return PF_EDEVICE;
-    if (hipHostMalloc(&h_tab_, sizeof(StepTable), hipHostMallocDefault) != hipSuccess) return PF_ENOMEM;
+    if (hipHostMalloc(&h_tab_, sizeof(StepTable), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
+        return PF_ENOMEM;
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&h_dev_), h_tab_, 0) != hipSuccess) return PF_EDEVICE;
     if (hipMalloc(&d_tab_, sizeof(StepTable)) != hipSuccess) return PF_ENOMEM;
     // the slot accumulators and the table counters live in the step table's
     // device-written part, which starts at zero
@@ -114,16 +116,15 @@ pf_status Server::start() {
 pf_status Server::build_graph(int rounds) {
     hipStream_t st = e_.stream;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return PF_EDEVICE;
-    // the host's part of the table in, the device's part (outcomes, slot
-    // accumulators, table counters) out: one copy each way
-    constexpr size_t kHostPart = offsetof(StepTable, out);
-    hipError_t e = hipMemcpyAsync(d_tab_, h_tab_, kHostPart, hipMemcpyHostToDevice, st);
+    // the used part of the host's table in, the outcomes (segments, roots,
+    // slot accumulators, table counters) out, each by a kernel that reads or
+    // writes the coherent host table: no copy node between the host and the
+    // step, and a small step moves only what it uses
+    hipError_t e = launch_fetch_step(d_tab_, h_dev_, st);
     if (e == hipSuccess)
         e = launch_step(e_.backend, e_.d_tab, e_.d_z, d_tab_, e_.d_tt, e_.tt_buckets ? e_.tt_mask_word() : 0, grid_,
                         up_grid_, rounds, st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(reinterpret_cast<char*>(h_tab_) + kHostPart, reinterpret_cast<char*>(d_tab_) + kHostPart,
-                           sizeof(StepTable) - kHostPart, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = launch_post_step(d_tab_, h_dev_, (u32)max_jobs_, st);
     // the capture is ended whatever happened inside it, so the stream leaves capture mode
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(st, &g);
@@ -176,7 +177,7 @@ void Server::stop() {
     hipFree(root_p_);
     hipHostFree(h_tab_);
     d_tab_ = nullptr; root_b_ = nullptr; root_w_ = nullptr; root_k1_ = nullptr; root_k2_ = nullptr;
-    root_p_ = nullptr; slot_acc_ = nullptr; h_tab_ = nullptr;
+    root_p_ = nullptr; slot_acc_ = nullptr; h_tab_ = nullptr; h_dev_ = nullptr;
 }
 
 // Predicted peak frontier bytes of a request: the written levels are plies

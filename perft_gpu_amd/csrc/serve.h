@@ -85,7 +85,8 @@ class Server {
     std::map<u64, u64> free_mem_;  // arena offset -> bytes of each free block (coalesced)
     u64 max_region_ = 0;
     // device / pinned resources
-    StepTable* h_tab_ = nullptr;
+    StepTable* h_tab_ = nullptr;  // coherent host memory, read and written by the step's first and last kernels
+    StepTable* h_dev_ = nullptr;  // h_tab_ as the device addresses it
     StepTable* d_tab_ = nullptr;
     Board* root_b_ = nullptr;
     u32* root_w_ = nullptr;

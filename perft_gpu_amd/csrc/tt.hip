@@ -27,6 +27,9 @@ constexpr int kTWG = 256;
 // average), so a tile's moves mostly fit one 4096-entry window; 128 had
 // about 4,200 and spilled a few dozen into a second window almost every tile.
 constexpr int kTTP = kTTTileParents;
+// Serving's late leaf passes split a small level into tiles of as few as this
+// many parents (about 30 children each: one child per thread of a workgroup).
+constexpr u32 kLateMinParents = 8;
 constexpr int kTCAP = 4096; // LDS move-list window
 // Move-list windows a tile can need (a position has at most 218 legal moves).
 constexpr int kTTMaxWin = 8;
@@ -702,26 +705,37 @@ template <int B, int W>
 __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void k_step_late(const SliderTables* tabs, const ZTables* zt, StepTable* t) {
     __shared__ TTLds s;
     __shared__ u32 pre[kMaxLate + 1];
-    __shared__ u32 s_tile, s_seg;
+    __shared__ u32 s_tile, s_seg, s_tp;
     const u32 nl = t->nlate;
     if (!nl) return;
     const auto sl = SlOf<B>::make(tabs);
     if (threadIdx.x < 64) {
-        u32 tiles = 0;
+        u32 n = 0;
         if (threadIdx.x < nl) {
             const TTArgs& w = t->seg[t->late_pos[threadIdx.x]];
-            const u32 n = *w.err ? 0u : *w.out_count;
-            tiles = (n + kTTP - 1) / kTTP;
+            n = *w.err ? 0u : *w.out_count;
         }
-        const u32 incl = wave_incl_scan(tiles);
+        // parents per tile: the fewest that still give every tile a workgroup
+        // of this launch (these levels are small and wait on the chain of one
+        // tile's dependent steps), at most a full tile
+        u32 tp = kTTP;
+        for (u32 c = kLateMinParents; c < (u32)kTTP; c *= 2)
+            if (__builtin_amdgcn_readlane(wave_incl_scan((n + c - 1) / c), 63) <= gridDim.x) {
+                tp = c;
+                break;
+            }
+        const u32 incl = wave_incl_scan((n + tp - 1) / tp);
         if (threadIdx.x < nl) pre[threadIdx.x + 1] = incl;
-        if (threadIdx.x == 0) pre[0] = 0;
+        if (threadIdx.x == 0) {
+            pre[0] = 0;
+            s_tp = tp;
+        }
     }
     __syncthreads();
-    const u32 total = pre[nl];
+    const u32 total = pre[nl], tp = s_tp;
     if (blockIdx.x >= total) return;  // more workgroups than tiles: the others drain the queue
-    u32 cur = 0xFFFFFFFFu;
-    TTArgs l = {};
+    u32 cur = 0xFFFFFFFFu, wn = 0;
+    TTArgs l = {}, wl = {};
     for (;;) {
         if (threadIdx.x == 0) {
             const u32 g = atomicAdd(&t->tile_ctr3, 1u);
@@ -739,16 +753,20 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
         if (g >= total) break;
         if (k != cur) {  // the write's children as parents (uniform: from LDS and the table)
             const TTArgs& w = t->seg[t->late_pos[k]];
-            l.in = w.out;
-            l.in_root = w.out_root;
-            l.in_acc = w.out_acc;
-            l.n_in = *w.out_count;
+            wl = w;
+            wn = *w.out_count;
             l.mode = kLeaf;
             l.child_depth = w.child_depth - 1;
             l.err = w.err;
             cur = k;
         }
-        tt_tile(sl, l, g - pre[k], zt, s);
+        // tile g - pre[k] of the level: its tp parents as a one-tile range
+        const u32 c0 = (g - pre[k]) * tp;
+        l.in = wl.out + c0;
+        l.in_root = wl.out_root + c0;
+        l.in_acc = wl.out_acc + c0;
+        l.n_in = min(tp, wn - c0);
+        tt_tile(sl, l, 0, zt, s);
     }
 }
 
@@ -810,6 +828,68 @@ __global__ __launch_bounds__(256) void k_up_step(const StepTable* __restrict__ t
         stores = wave_sum64(stores);
         if ((threadIdx.x & 63) == 0 && stores) atomicAdd(t->stores, stores);
     }
+}
+
+// Serving: the step table's host part, read by the GPU from coherent host
+// memory -- only what the step uses (header, tile prefix, upsweep rounds and
+// the used segments, upsweeps and roots), so that a small step moves a few
+// hundred bytes rather than the whole table, and no copy node sits between
+// the host and the step's first kernel.  Copied as 4-byte words.
+__global__ __launch_bounds__(256) void k_fetch_step(StepTable* __restrict__ d, const StepTable* __restrict__ h) {
+    __shared__ u32 hdr[3];
+    if (threadIdx.x == 0) {
+        hdr[0] = h->nseg;
+        hdr[1] = h->nup;
+        hdr[2] = h->nroot;
+    }
+    __syncthreads();
+    // (clamped to the table's capacities, so every word copied lies inside it)
+    const u32 nseg = min(hdr[0], (u32)kMaxSegs), nup = min(hdr[1], (u32)kMaxUps), nroot = min(hdr[2], (u32)kMaxRoots);
+    constexpr u32 kHdr = offsetof(StepTable, tile_prefix) / 4;
+    const u32 o_tp = offsetof(StepTable, tile_prefix) / 4, n_tp = nseg + 1;
+    const u32 o_ur = offsetof(StepTable, up_round) / 4, n_ur = kUpRounds + 1;
+    const u32 o_sg = offsetof(StepTable, seg) / 4, n_sg = nseg * (u32)(sizeof(TTArgs) / 4);
+    const u32 o_up = offsetof(StepTable, up) / 4, n_up = nup * (u32)(sizeof(UpSeg) / 4);
+    const u32 o_rt = offsetof(StepTable, root) / 4, n_rt = nroot * (u32)(sizeof(RootSeg) / 4);
+    const u32 total = kHdr + n_tp + n_ur + n_sg + n_up + n_rt;
+    const u32* src = reinterpret_cast<const u32*>(h);
+    u32* dst = reinterpret_cast<u32*>(d);
+    for (u32 i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256) {
+        u32 k = i, w;  // word k of the concatenated ranges -> its word in the table
+        if (k < kHdr) w = k;
+        else if ((k -= kHdr) < n_tp) w = o_tp + k;
+        else if ((k -= n_tp) < n_ur) w = o_ur + k;
+        else if ((k -= n_ur) < n_sg) w = o_sg + k;
+        else if ((k -= n_sg) < n_up) w = o_up + k;
+        else w = o_rt + (k - n_up);
+        dst[w] = src[w];
+    }
+}
+static_assert(sizeof(TTArgs) % 4 == 0 && sizeof(UpSeg) % 4 == 0 && sizeof(RootSeg) % 4 == 0 &&
+                  offsetof(StepTable, tile_prefix) % 4 == 0,
+              "the step table is fetched in 4-byte words");
+
+// Serving: the step's outcomes back into the coherent host table (the used
+// segment outcomes, root outcomes, the slot accumulators and the table
+// counters), made visible to the host before the step completes.
+__global__ __launch_bounds__(256) void k_post_step(const StepTable* d, StepTable* h, u32 max_jobs) {
+    const u32 nseg = min(d->nseg, (u32)kMaxSegs), nroot = min(d->nroot, (u32)kMaxRoots);
+    max_jobs = min(max_jobs, (u32)kMaxRoots);
+    const u32 stride = gridDim.x * 256;
+    for (u32 i = blockIdx.x * 256 + threadIdx.x; i < nseg; i += stride) h->out[i] = d->out[i];
+    for (u32 i = blockIdx.x * 256 + threadIdx.x; i < nroot; i += stride) h->rout[i] = d->rout[i];
+    for (u32 i = blockIdx.x * 256 + threadIdx.x; i < max_jobs; i += stride) h->acc[i] = d->acc[i];
+    if (blockIdx.x == 0 && threadIdx.x < 2) h->stats[threadIdx.x] = d->stats[threadIdx.x];
+    __threadfence_system();
+}
+
+hipError_t launch_fetch_step(StepTable* d, const StepTable* h, hipStream_t s) {
+    hipLaunchKernelGGL(k_fetch_step, dim3(32), dim3(256), 0, s, d, h);
+    return hipGetLastError();
+}
+hipError_t launch_post_step(const StepTable* d, StepTable* h, u32 max_jobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_post_step, dim3(4), dim3(256), 0, s, d, h, max_jobs);
+    return hipGetLastError();
 }
 
 // Start of a step: the step's segment outcomes zeroed, new requests (table

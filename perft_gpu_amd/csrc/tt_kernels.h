@@ -155,8 +155,8 @@ struct StepTable {
     TTArgs seg[kMaxSegs];
     UpSeg up[kMaxUps];
     RootSeg root[kMaxRoots];
-    // device-written part, from out[] on: the host copies in everything
-    // before it and reads back everything from it, one copy each way
+    // device-written part, from out[] on: serving's fetch kernel copies in
+    // the used entries before it, its post kernel the used entries from it
     SegOut out[kMaxSegs];          // zeroed by the start kernel for [0, nseg)
     RootOut rout[kMaxRoots];
     unsigned long long acc[kMaxRoots];  // serving: each job slot's root accumulator
@@ -223,6 +223,10 @@ hipError_t capture_plan_pass(int backend, const SliderTables* tabs, const ZTable
                              u64 mask, unsigned long long* ttstats, int r, int write_grid, int leaf_grid,
                              int up_grid, hipStream_t s);
 
+// Serving: the used part of the step table from coherent host memory `h`
+// into `d`; the step's outcomes from `d` back into `h`.
+hipError_t launch_fetch_step(StepTable* d, const StepTable* h, hipStream_t s);
+hipError_t launch_post_step(const StepTable* d, StepTable* h, u32 max_jobs, hipStream_t s);
 int step_grid(int backend, int n_cu);       // workgroups of the serving step's mixed launch
 int step_max_bpc(int backend, int leaf);  // the planned pass's write (0) / leaf (1) step kernel
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
