@@ -113,7 +113,7 @@ pf_status Server::start() {
 // One graph per number of upsweep rounds a step needs (0..kUpRounds): a step
 // replays the one with as many rounds as its upsweep chains fill, so a small
 // step does not pay for empty upsweep launches.
-pf_status Server::build_graph(int rounds) {
+pf_status Server::build_graph(int fine, int rounds) {
     hipStream_t st = e_.stream;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return PF_EDEVICE;
     // the used part of the host's table in, the outcomes (segments, roots,
@@ -123,7 +123,7 @@ pf_status Server::build_graph(int rounds) {
     hipError_t e = launch_fetch_step(d_tab_, h_dev_, st);
     if (e == hipSuccess)
         e = launch_step(e_.backend, e_.d_tab, e_.d_z, d_tab_, e_.d_tt, e_.tt_buckets ? e_.tt_mask_word() : 0, grid_,
-                        up_grid_, rounds, st);
+                        up_grid_, rounds, fine, st);
     if (e == hipSuccess) e = launch_post_step(d_tab_, h_dev_, (u32)max_jobs_, st);
     // the capture is ended whatever happened inside it, so the stream leaves capture mode
     hipGraph_t g = nullptr;
@@ -132,8 +132,8 @@ pf_status Server::build_graph(int rounds) {
         if (g) hipGraphDestroy(g);
         return PF_EDEVICE;
     }
-    graph_[rounds] = g;
-    if (hipGraphInstantiate(&exec_[rounds], g, nullptr, nullptr, 0) != hipSuccess) return PF_EDEVICE;
+    graph_[fine][rounds] = g;
+    if (hipGraphInstantiate(&exec_[fine][rounds], g, nullptr, nullptr, 0) != hipSuccess) return PF_EDEVICE;
     return PF_OK;
 }
 
@@ -163,12 +163,13 @@ void Server::stop() {
         std::lock_guard<std::mutex> g(mu_);
         running_ = false;
     }
-    for (int r = 0; r <= kUpRounds; r++) {
-        if (exec_[r]) hipGraphExecDestroy(exec_[r]);
-        if (graph_[r]) hipGraphDestroy(graph_[r]);
-        exec_[r] = nullptr;
-        graph_[r] = nullptr;
-    }
+    for (int f = 0; f < 2; f++)
+        for (int r = 0; r <= kUpRounds; r++) {
+            if (exec_[f][r]) hipGraphExecDestroy(exec_[f][r]);
+            if (graph_[f][r]) hipGraphDestroy(graph_[f][r]);
+            exec_[f][r] = nullptr;
+            graph_[f][r] = nullptr;
+        }
     hipFree(d_tab_);
     hipFree(root_b_);
     hipFree(root_w_);
@@ -573,7 +574,8 @@ bool Server::plan(Job& j) {
 void Server::run() {
     {
         pf_status gs = hipSetDevice(e_.device) == hipSuccess ? PF_OK : PF_EDEVICE;
-        for (int r = 0; r <= kUpRounds && gs == PF_OK; r++) gs = build_graph(r);
+        for (int f = 0; f < 2; f++)
+            for (int r = 0; r <= kUpRounds && gs == PF_OK; r++) gs = build_graph(f, r);
         std::lock_guard<std::mutex> g(mu_);
         graph_status_ = gs;
     }
@@ -699,11 +701,25 @@ void Server::run() {
             t.tile_ctr2 = 0;
             for (u32 k = 0; k < t.nlate; k++) t.late_pos[k] = pos_of_[late_src_[k]];
         }
-        // tile prefix over expand segments
+        // tile prefix over expand segments; a step whose levels are small
+        // tiles them finer (parents per tile: the fewest from kFineMinParents
+        // up that give every tile a workgroup of the mixed launch), since its
+        // time is then one tile's chain of dependent steps
+        u32 tp = kTTTileParents;
+        for (u32 c = kFineMinParents; c < (u32)kTTTileParents; c *= 2) {
+            u64 n = 0;
+            for (u32 i = 0; i < t.nseg; i++) n += (t.seg[i].n_in + c - 1) / c;
+            if (n <= (u64)grid_) {
+                tp = c;
+                break;
+            }
+        }
+        const int fine = tp < (u32)kTTTileParents;
+        t.mix_tp = tp;
         u32 acc_t = 0;
         for (u32 i = 0; i < t.nseg; i++) {
             t.tile_prefix[i] = acc_t;
-            acc_t += (t.seg[i].n_in + kTTTileParents - 1) / kTTTileParents;
+            acc_t += (t.seg[i].n_in + tp - 1) / tp;
         }
         t.tile_prefix[t.nseg] = acc_t;
         t.mix_write_wgs = mix_write_wgs(t.tile_prefix[t.nseg_write], acc_t - t.tile_prefix[t.nseg_write]);
@@ -724,7 +740,7 @@ void Server::run() {
         int rounds = 0;  // rounds are filled in order: round r + 1 only after round r
         while (rounds < kUpRounds && t.up_round[rounds + 1] > t.up_round[rounds]) rounds++;
         const bool ok =
-            hipGraphLaunch(exec_[rounds], e_.stream) == hipSuccess && hipStreamSynchronize(e_.stream) == hipSuccess;
+            hipGraphLaunch(exec_[fine][rounds], e_.stream) == hipSuccess && hipStreamSynchronize(e_.stream) == hipSuccess;
         const double t1 = now_s();
         {
             std::lock_guard<std::mutex> g(mu_);

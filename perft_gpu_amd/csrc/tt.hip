@@ -27,9 +27,6 @@ constexpr int kTWG = 256;
 // average), so a tile's moves mostly fit one 4096-entry window; 128 had
 // about 4,200 and spilled a few dozen into a second window almost every tile.
 constexpr int kTTP = kTTTileParents;
-// Serving's late leaf passes split a small level into tiles of as few as this
-// many parents (about 30 children each: one child per thread of a workgroup).
-constexpr u32 kLateMinParents = 8;
 constexpr int kTCAP = 4096; // LDS move-list window
 // Move-list windows a tile can need (a position has at most 218 legal moves).
 constexpr int kTTMaxWin = 8;
@@ -640,7 +637,7 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
 // nseg) with tile_ctr2; their bounds are read once, before the loop, and the
 // tile prefix is copied to LDS, so that taking a tile is one atomic and a
 // search in LDS (small steps are latency-bound).
-template <int B, int W>
+template <int B, int W, bool F>
 __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void k_step_mixed(const SliderTables* tabs, const ZTables* zt, StepTable* t) {
     __shared__ TTLds s;
     __shared__ TTArgs sa;
@@ -650,6 +647,7 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
     const u32 nw = t->nseg_write, ns = t->nseg;
     const u32 wt = t->tile_prefix[nw], lt = t->tile_prefix[ns];  // write tiles [0, wt), leaf tiles [wt, lt)
     if (blockIdx.x >= lt) return;  // more workgroups than tiles: the others drain the queues
+    const u32 tp = F ? t->mix_tp : (u32)kTTP;
     for (u32 k = threadIdx.x; k <= ns; k += kTWG) pre[k] = t->tile_prefix[k];
     __syncthreads();
     bool leaf = blockIdx.x >= t->mix_write_wgs;  // thread 0's queue
@@ -692,7 +690,22 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
         }
         __syncthreads();
         const TTArgs a = uniform_args(sa);  // the segment's own mode
-        tt_tile(sl, a, g - pre[seg], zt, s);
+        if (F) {  // tile g - pre[seg] of the segment: its tp parents as a one-tile range
+            const u32 c0 = (g - pre[seg]) * tp;
+            TTArgs f = a;
+            f.in += c0;
+            f.in_root += c0;
+            f.in_acc += c0;
+            f.in_base += c0;
+            if (f.in_k1) {
+                f.in_k1 += c0;
+                f.in_k2 += c0;
+            }
+            f.n_in = min(tp, a.n_in - c0);
+            tt_tile(sl, f, 0, zt, s);
+        } else {
+            tt_tile(sl, a, g - pre[seg], zt, s);
+        }
     }
 }
 
@@ -719,7 +732,7 @@ __global__ __launch_bounds__(kTWG) __attribute__((amdgpu_waves_per_eu(W))) void 
         // of this launch (these levels are small and wait on the chain of one
         // tile's dependent steps), at most a full tile
         u32 tp = kTTP;
-        for (u32 c = kLateMinParents; c < (u32)kTTP; c *= 2)
+        for (u32 c = kFineMinParents; c < (u32)kTTP; c *= 2)
             if (__builtin_amdgcn_readlane(wave_incl_scan((n + c - 1) / c), 63) <= gridDim.x) {
                 tp = c;
                 break;
@@ -960,10 +973,10 @@ int step_max_bpc(int backend, int leaf) {
 
 // The serving step's mixed kernel: its most resident workgroups per CU x CUs.
 int step_grid(int backend, int n_cu) {
-    const void* f = backend == kKS ? (const void*)&k_step_mixed<kKS, kTTMixedWaves>
-                    : backend == kPlain ? (const void*)&k_step_mixed<kPlain, kTTMixedWaves>
-                    : backend == kFancy ? (const void*)&k_step_mixed<kFancy, kTTMixedWaves>
-                                        : (const void*)&k_step_mixed<kByte, kTTMixedWaves>;
+    const void* f = backend == kKS ? (const void*)&k_step_mixed<kKS, kTTMixedWaves, false>
+                    : backend == kPlain ? (const void*)&k_step_mixed<kPlain, kTTMixedWaves, false>
+                    : backend == kFancy ? (const void*)&k_step_mixed<kFancy, kTTMixedWaves, false>
+                                        : (const void*)&k_step_mixed<kByte, kTTMixedWaves, false>;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kTWG, 0) != hipSuccess || per_cu < 1) {
         (void)hipGetLastError();
@@ -977,30 +990,33 @@ int step_grid(int backend, int n_cu) {
 // (<= kUpRounds) upsweep rounds.  up_grid * 256 must cover
 // kMaxRoots and kMaxSegs (the roots and the outcomes are handled by the first
 // threads of the start kernel).
+// The step's start, mixed and late launches for one backend and tiling.
+template <int B, bool F>
+static void launch_step_front(const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask, int grid,
+                              int up_grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_pre_step<B>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
+    hipLaunchKernelGGL((k_step_mixed<B, kTTMixedWaves, F>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+    hipLaunchKernelGGL((k_step_late<B, kTTLeafWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+}
+
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
-                       int grid, int up_grid, int up_rounds, hipStream_t s) {
+                       int grid, int up_grid, int up_rounds, int fine, hipStream_t s) {
     if (up_grid * 256 < kMaxRoots || up_grid * 256 < kMaxSegs) return hipErrorInvalidValue;
     if (up_rounds < 0 || up_rounds > kUpRounds) return hipErrorInvalidValue;
     switch (backend) {
     case kKS:
-        hipLaunchKernelGGL(k_pre_step<kKS>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
-        hipLaunchKernelGGL((k_step_mixed<kKS, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
-        hipLaunchKernelGGL((k_step_late<kKS, kTTLeafWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+        (fine ? launch_step_front<kKS, true> : launch_step_front<kKS, false>)(tabs, zt, t, tt, mask, grid, up_grid, s);
         break;
     case kPlain:
-        hipLaunchKernelGGL(k_pre_step<kPlain>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
-        hipLaunchKernelGGL((k_step_mixed<kPlain, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
-        hipLaunchKernelGGL((k_step_late<kPlain, kTTLeafWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+        (fine ? launch_step_front<kPlain, true> : launch_step_front<kPlain, false>)(tabs, zt, t, tt, mask, grid, up_grid,
+                                                                                   s);
         break;
     case kFancy:
-        hipLaunchKernelGGL(k_pre_step<kFancy>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
-        hipLaunchKernelGGL((k_step_mixed<kFancy, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
-        hipLaunchKernelGGL((k_step_late<kFancy, kTTLeafWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+        (fine ? launch_step_front<kFancy, true> : launch_step_front<kFancy, false>)(tabs, zt, t, tt, mask, grid, up_grid,
+                                                                                   s);
         break;
     default:
-        hipLaunchKernelGGL(k_pre_step<kByte>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
-        hipLaunchKernelGGL((k_step_mixed<kByte, kTTMixedWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
-        hipLaunchKernelGGL((k_step_late<kByte, kTTLeafWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+        (fine ? launch_step_front<kByte, true> : launch_step_front<kByte, false>)(tabs, zt, t, tt, mask, grid, up_grid, s);
         break;
     }
     for (int r = 0; r < up_rounds; r++) hipLaunchKernelGGL(k_up_step, dim3(up_grid), dim3(256), 0, s, t, tt, mask, r);

@@ -86,6 +86,10 @@ constexpr int kMaxUps = 512;
 constexpr int kMaxRoots = 256;
 // Serving: leaf passes per step over levels written in the same step.
 constexpr int kMaxLate = 16;
+// Serving: a step whose levels are small splits them into tiles of as few as
+// this many parents (about 30 children each: one child per thread of a
+// workgroup), the fewest that still give every tile a workgroup of the launch.
+constexpr u32 kFineMinParents = 8;
 
 struct SegOut {
     u32 out_count;
@@ -139,6 +143,9 @@ struct StepTable {
     // the serving step's mixed launch: workgroups [0, mix_write_wgs) start on
     // the write tiles, the others on the leaf tiles
     u32 mix_write_wgs;
+    // parents per tile of the mixed launch's finely tiled form (its tile
+    // prefix counts with it); the other form tiles by kTTTileParents
+    u32 mix_tp;
     // planned pass: the last write level runs its leaf pass fused (set by its
     // planner when the level has enough tiles for every workgroup)
     u32 fuse_leaf;
@@ -229,8 +236,9 @@ hipError_t launch_fetch_step(StepTable* d, const StepTable* h, hipStream_t s);
 hipError_t launch_post_step(const StepTable* d, StepTable* h, u32 max_jobs, hipStream_t s);
 int step_grid(int backend, int n_cu);       // workgroups of the serving step's mixed launch
 int step_max_bpc(int backend, int leaf);  // the planned pass's write (0) / leaf (1) step kernel
+// fine: the mixed launch tiles by StepTable::mix_tp parents, not kTTTileParents
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
-                       int grid, int up_grid, int up_rounds, hipStream_t s);
+                       int grid, int up_grid, int up_rounds, int fine, hipStream_t s);
 int expand_tt_max_bpc(int backend);   // most resident workgroups per CU of the TT write-level kernel
 int expand_tt_leaf_bpc(int backend);  // the same for the TT leaf-level kernel
 hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables* zt, const TTArgs& a, int grid,
