@@ -137,6 +137,23 @@ pf_status Server::build_graph(int fine, int rounds) {
     return PF_OK;
 }
 
+// Waits for step `seq`. The step's last kernel writes seq to the host table's
+// done word after every outcome has reached the host, a few microseconds
+// before the stream's completion would wake a waiting thread, so the serving
+// thread watches that word. A step still running after kSpinWaitS is waited
+// for by hipStreamSynchronize, which also reports a failed launch.
+bool Server::wait_step(u32 seq) {
+    constexpr double kSpinWaitS = 0.001;
+    const double t0 = now_s();
+    while (__atomic_load_n(&h_tab_->done, __ATOMIC_ACQUIRE) != seq) {
+        if (now_s() - t0 > kSpinWaitS)
+            return hipStreamSynchronize(e_.stream) == hipSuccess &&
+                   __atomic_load_n(&h_tab_->done, __ATOMIC_ACQUIRE) == seq;
+        __builtin_ia32_pause();
+    }
+    return true;
+}
+
 // Workgroups of the mixed step launch that start on the write tiles: a share
 // of the workgroups that get a tile (those past the step's tile count leave
 // at once) proportional to the write tiles' part of the step's work, a write
@@ -739,8 +756,8 @@ void Server::run() {
         const double t0 = now_s();
         int rounds = 0;  // rounds are filled in order: round r + 1 only after round r
         while (rounds < kUpRounds && t.up_round[rounds + 1] > t.up_round[rounds]) rounds++;
-        const bool ok =
-            hipGraphLaunch(exec_[fine][rounds], e_.stream) == hipSuccess && hipStreamSynchronize(e_.stream) == hipSuccess;
+        t.seq = ++seq_;
+        const bool ok = hipGraphLaunch(exec_[fine][rounds], e_.stream) == hipSuccess && wait_step(t.seq);
         const double t1 = now_s();
         {
             std::lock_guard<std::mutex> g(mu_);
@@ -819,6 +836,9 @@ void Server::run() {
             dups_ += step_dups;
         }
     }
+    // the last step's outcomes were seen in the host table; its graph is
+    // complete on the stream too before the server's memory is freed
+    hipStreamSynchronize(e_.stream);
 }
 
 }  // namespace pf

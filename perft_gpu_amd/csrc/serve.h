@@ -65,6 +65,7 @@ class Server {
     char* take(Job& j, u64 bytes);
     bool alloc_level(Job& j, u64 n, Engine::TLevel* lv);
     pf_status build_graph(int fine, int rounds);
+    bool wait_step(u32 seq);
     u32 mix_write_wgs(u32 write_tiles, u32 leaf_tiles) const;
 
     Engine& e_;
@@ -99,6 +100,7 @@ class Server {
     std::vector<TTArgs> seg_tmp_;  // reordering buffer of a step's segments
     std::vector<u32> pos_of_;      // a segment's position after the reordering
     bool late_ok_ = false;         // this step may hold late leaf passes
+    u32 seq_ = 0;                  // steps launched
     u32 late_src_[kMaxLate] = {};  // their write segments (indices before the reordering)
     hipGraph_t graph_[2][kUpRounds + 1] = {};     // by tiling (1: fine), then by upsweep rounds
     hipGraphExec_t exec_[2][kUpRounds + 1] = {};

@@ -884,16 +884,20 @@ static_assert(sizeof(TTArgs) % 4 == 0 && sizeof(UpSeg) % 4 == 0 && sizeof(RootSe
 
 // Serving: the step's outcomes back into the coherent host table (the used
 // segment outcomes, root outcomes, the slot accumulators and the table
-// counters), made visible to the host before the step completes.
+// counters); then, once all of them are visible to the host, the step's
+// number into done, so that the host can wait for the step by watching it.
+// One workgroup.
 __global__ __launch_bounds__(256) void k_post_step(const StepTable* d, StepTable* h, u32 max_jobs) {
     const u32 nseg = min(d->nseg, (u32)kMaxSegs), nroot = min(d->nroot, (u32)kMaxRoots);
     max_jobs = min(max_jobs, (u32)kMaxRoots);
-    const u32 stride = gridDim.x * 256;
-    for (u32 i = blockIdx.x * 256 + threadIdx.x; i < nseg; i += stride) h->out[i] = d->out[i];
-    for (u32 i = blockIdx.x * 256 + threadIdx.x; i < nroot; i += stride) h->rout[i] = d->rout[i];
-    for (u32 i = blockIdx.x * 256 + threadIdx.x; i < max_jobs; i += stride) h->acc[i] = d->acc[i];
-    if (blockIdx.x == 0 && threadIdx.x < 2) h->stats[threadIdx.x] = d->stats[threadIdx.x];
-    __threadfence_system();
+    for (u32 i = threadIdx.x; i < nseg; i += 256) h->out[i] = d->out[i];
+    for (u32 i = threadIdx.x; i < nroot; i += 256) h->rout[i] = d->rout[i];
+    for (u32 i = threadIdx.x; i < max_jobs; i += 256) h->acc[i] = d->acc[i];
+    if (threadIdx.x < 2) h->stats[threadIdx.x] = d->stats[threadIdx.x];
+    __threadfence_system();  // this thread's writes reach the host before the barrier ...
+    __syncthreads();
+    if (threadIdx.x == 0)    // ... and so before done
+        __hip_atomic_store(&h->done, d->seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 hipError_t launch_fetch_step(StepTable* d, const StepTable* h, hipStream_t s) {
@@ -901,7 +905,7 @@ hipError_t launch_fetch_step(StepTable* d, const StepTable* h, hipStream_t s) {
     return hipGetLastError();
 }
 hipError_t launch_post_step(const StepTable* d, StepTable* h, u32 max_jobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_post_step, dim3(4), dim3(256), 0, s, d, h, max_jobs);
+    hipLaunchKernelGGL(k_post_step, dim3(1), dim3(256), 0, s, d, h, max_jobs);
     return hipGetLastError();
 }
 

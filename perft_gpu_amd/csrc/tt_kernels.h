@@ -154,6 +154,7 @@ struct StepTable {
     // seg[]; their tiles come from tile_ctr3
     u32 nlate, tile_ctr3;
     u32 late_pos[kMaxLate];
+    u32 seq;  // serving: the step's number, which the post kernel writes to done last
     unsigned long long* stores;    // table stores of the upsweeps are counted here (null: not counted)
     u32 tile_prefix[kMaxSegs + 1];
     u32 up_prefix[kMaxUps + 1];    // item prefix over all upsweep segments (the upsweep kernel recomputes
@@ -168,6 +169,7 @@ struct StepTable {
     RootOut rout[kMaxRoots];
     unsigned long long acc[kMaxRoots];  // serving: each job slot's root accumulator
     unsigned long long stats[2];        // serving: table probes and hits of the write segments
+    u32 done, pad;                      // serving: seq of the last step whose outcomes are all in the host table
 };
 
 // ---- device-planned table pass ----------------------------------------------
@@ -231,7 +233,8 @@ hipError_t capture_plan_pass(int backend, const SliderTables* tabs, const ZTable
                              int up_grid, hipStream_t s);
 
 // Serving: the used part of the step table from coherent host memory `h`
-// into `d`; the step's outcomes from `d` back into `h`.
+// into `d`; the step's outcomes from `d` back into `h`, then h->done = d->seq
+// once every outcome is visible to the host.
 hipError_t launch_fetch_step(StepTable* d, const StepTable* h, hipStream_t s);
 hipError_t launch_post_step(const StepTable* d, StepTable* h, u32 max_jobs, hipStream_t s);
 int step_grid(int backend, int n_cu);       // workgroups of the serving step's mixed launch
