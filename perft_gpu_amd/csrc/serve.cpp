@@ -110,10 +110,10 @@ pf_status Server::start() {
     return PF_OK;
 }
 
-// One graph per number of upsweep rounds a step needs (0..kUpRounds): a step
-// replays the one with as many rounds as its upsweep chains fill, so a small
-// step does not pay for empty upsweep launches.
-pf_status Server::build_graph(int fine, int rounds) {
+// One graph per form of the mixed launch (fine), presence of late leaf passes
+// (late) and number of upsweep rounds (0..kUpRounds): a step replays the one
+// that fits it, so a small step does not pay for launches it leaves empty.
+pf_status Server::build_graph(int fine, int late, int rounds) {
     hipStream_t st = e_.stream;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return PF_EDEVICE;
     // the used part of the host's table in, the outcomes (segments, roots,
@@ -123,7 +123,7 @@ pf_status Server::build_graph(int fine, int rounds) {
     hipError_t e = launch_fetch_step(d_tab_, h_dev_, st);
     if (e == hipSuccess)
         e = launch_step(e_.backend, e_.d_tab, e_.d_z, d_tab_, e_.d_tt, e_.tt_buckets ? e_.tt_mask_word() : 0, grid_,
-                        up_grid_, rounds, fine, st);
+                        up_grid_, rounds, fine, late, st);
     if (e == hipSuccess) e = launch_post_step(d_tab_, h_dev_, (u32)max_jobs_, st);
     // the capture is ended whatever happened inside it, so the stream leaves capture mode
     hipGraph_t g = nullptr;
@@ -132,8 +132,8 @@ pf_status Server::build_graph(int fine, int rounds) {
         if (g) hipGraphDestroy(g);
         return PF_EDEVICE;
     }
-    graph_[fine][rounds] = g;
-    if (hipGraphInstantiate(&exec_[fine][rounds], g, nullptr, nullptr, 0) != hipSuccess) return PF_EDEVICE;
+    graph_[fine][late][rounds] = g;
+    if (hipGraphInstantiate(&exec_[fine][late][rounds], g, nullptr, nullptr, 0) != hipSuccess) return PF_EDEVICE;
     return PF_OK;
 }
 
@@ -181,12 +181,13 @@ void Server::stop() {
         running_ = false;
     }
     for (int f = 0; f < 2; f++)
-        for (int r = 0; r <= kUpRounds; r++) {
-            if (exec_[f][r]) hipGraphExecDestroy(exec_[f][r]);
-            if (graph_[f][r]) hipGraphDestroy(graph_[f][r]);
-            exec_[f][r] = nullptr;
-            graph_[f][r] = nullptr;
-        }
+        for (int l = 0; l < 2; l++)
+            for (int r = 0; r <= kUpRounds; r++) {
+                if (exec_[f][l][r]) hipGraphExecDestroy(exec_[f][l][r]);
+                if (graph_[f][l][r]) hipGraphDestroy(graph_[f][l][r]);
+                exec_[f][l][r] = nullptr;
+                graph_[f][l][r] = nullptr;
+            }
     hipFree(d_tab_);
     hipFree(root_b_);
     hipFree(root_w_);
@@ -592,7 +593,8 @@ void Server::run() {
     {
         pf_status gs = hipSetDevice(e_.device) == hipSuccess ? PF_OK : PF_EDEVICE;
         for (int f = 0; f < 2; f++)
-            for (int r = 0; r <= kUpRounds && gs == PF_OK; r++) gs = build_graph(f, r);
+            for (int l = 0; l < 2; l++)
+                for (int r = 0; r <= kUpRounds && gs == PF_OK; r++) gs = build_graph(f, l, r);
         std::lock_guard<std::mutex> g(mu_);
         graph_status_ = gs;
     }
@@ -757,7 +759,7 @@ void Server::run() {
         int rounds = 0;  // rounds are filled in order: round r + 1 only after round r
         while (rounds < kUpRounds && t.up_round[rounds + 1] > t.up_round[rounds]) rounds++;
         t.seq = ++seq_;
-        const bool ok = hipGraphLaunch(exec_[fine][rounds], e_.stream) == hipSuccess && wait_step(t.seq);
+        const bool ok = hipGraphLaunch(exec_[fine][t.nlate > 0][rounds], e_.stream) == hipSuccess && wait_step(t.seq);
         const double t1 = now_s();
         {
             std::lock_guard<std::mutex> g(mu_);

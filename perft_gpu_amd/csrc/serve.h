@@ -64,7 +64,7 @@ class Server {
     void finish(Job& j, u64 nodes, int status);
     char* take(Job& j, u64 bytes);
     bool alloc_level(Job& j, u64 n, Engine::TLevel* lv);
-    pf_status build_graph(int fine, int rounds);
+    pf_status build_graph(int fine, int late, int rounds);
     bool wait_step(u32 seq);
     u32 mix_write_wgs(u32 write_tiles, u32 leaf_tiles) const;
 
@@ -102,8 +102,9 @@ class Server {
     bool late_ok_ = false;         // this step may hold late leaf passes
     u32 seq_ = 0;                  // steps launched
     u32 late_src_[kMaxLate] = {};  // their write segments (indices before the reordering)
-    hipGraph_t graph_[2][kUpRounds + 1] = {};     // by tiling (1: fine), then by upsweep rounds
-    hipGraphExec_t exec_[2][kUpRounds + 1] = {};
+    // by tiling (1: fine), late leaf launch (1: recorded), then upsweep rounds
+    hipGraph_t graph_[2][2][kUpRounds + 1] = {};
+    hipGraphExec_t exec_[2][2][kUpRounds + 1] = {};
     u64 steps_ = 0, segs_ = 0, dups_ = 0;
     double busy_ = 0;
 };

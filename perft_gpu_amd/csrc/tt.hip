@@ -994,35 +994,28 @@ int step_grid(int backend, int n_cu) {
 // (<= kUpRounds) upsweep rounds.  up_grid * 256 must cover
 // kMaxRoots and kMaxSegs (the roots and the outcomes are handled by the first
 // threads of the start kernel).
-// The step's start, mixed and late launches for one backend and tiling.
+// The step's start, mixed and (late) late launches for one backend and tiling.
 template <int B, bool F>
 static void launch_step_front(const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask, int grid,
-                              int up_grid, hipStream_t s) {
+                              int up_grid, int late, hipStream_t s) {
     hipLaunchKernelGGL(k_pre_step<B>, dim3(up_grid), dim3(256), 0, s, tabs, t, t->seg, tt, mask);
     hipLaunchKernelGGL((k_step_mixed<B, kTTMixedWaves, F>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
-    hipLaunchKernelGGL((k_step_late<B, kTTLeafWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
+    if (late) hipLaunchKernelGGL((k_step_late<B, kTTLeafWaves>), dim3(grid), dim3(kTWG), 0, s, tabs, zt, t);
 }
 
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
-                       int grid, int up_grid, int up_rounds, int fine, hipStream_t s) {
+                       int grid, int up_grid, int up_rounds, int fine, int late, hipStream_t s) {
     if (up_grid * 256 < kMaxRoots || up_grid * 256 < kMaxSegs) return hipErrorInvalidValue;
     if (up_rounds < 0 || up_rounds > kUpRounds) return hipErrorInvalidValue;
+    using Front = void (*)(const SliderTables*, const ZTables*, StepTable*, uint4*, u64, int, int, int, hipStream_t);
+    Front front;
     switch (backend) {
-    case kKS:
-        (fine ? launch_step_front<kKS, true> : launch_step_front<kKS, false>)(tabs, zt, t, tt, mask, grid, up_grid, s);
-        break;
-    case kPlain:
-        (fine ? launch_step_front<kPlain, true> : launch_step_front<kPlain, false>)(tabs, zt, t, tt, mask, grid, up_grid,
-                                                                                   s);
-        break;
-    case kFancy:
-        (fine ? launch_step_front<kFancy, true> : launch_step_front<kFancy, false>)(tabs, zt, t, tt, mask, grid, up_grid,
-                                                                                   s);
-        break;
-    default:
-        (fine ? launch_step_front<kByte, true> : launch_step_front<kByte, false>)(tabs, zt, t, tt, mask, grid, up_grid, s);
-        break;
+    case kKS: front = fine ? launch_step_front<kKS, true> : launch_step_front<kKS, false>; break;
+    case kPlain: front = fine ? launch_step_front<kPlain, true> : launch_step_front<kPlain, false>; break;
+    case kFancy: front = fine ? launch_step_front<kFancy, true> : launch_step_front<kFancy, false>; break;
+    default: front = fine ? launch_step_front<kByte, true> : launch_step_front<kByte, false>; break;
     }
+    front(tabs, zt, t, tt, mask, grid, up_grid, late, s);
     for (int r = 0; r < up_rounds; r++) hipLaunchKernelGGL(k_up_step, dim3(up_grid), dim3(256), 0, s, t, tt, mask, r);
     return hipGetLastError();
 }

@@ -239,9 +239,10 @@ hipError_t launch_fetch_step(StepTable* d, const StepTable* h, hipStream_t s);
 hipError_t launch_post_step(const StepTable* d, StepTable* h, u32 max_jobs, hipStream_t s);
 int step_grid(int backend, int n_cu);       // workgroups of the serving step's mixed launch
 int step_max_bpc(int backend, int leaf);  // the planned pass's write (0) / leaf (1) step kernel
-// fine: the mixed launch tiles by StepTable::mix_tp parents, not kTTTileParents
+// fine: the mixed launch tiles by StepTable::mix_tp parents, not kTTTileParents;
+// late: the late leaf launch is recorded (steps with StepTable::nlate > 0)
 hipError_t launch_step(int backend, const SliderTables* tabs, const ZTables* zt, StepTable* t, uint4* tt, u64 mask,
-                       int grid, int up_grid, int up_rounds, int fine, hipStream_t s);
+                       int grid, int up_grid, int up_rounds, int fine, int late, hipStream_t s);
 int expand_tt_max_bpc(int backend);   // most resident workgroups per CU of the TT write-level kernel
 int expand_tt_leaf_bpc(int backend);  // the same for the TT leaf-level kernel
 hipError_t launch_expand_tt(int backend, const SliderTables* tabs, const ZTables* zt, const TTArgs& a, int grid,
