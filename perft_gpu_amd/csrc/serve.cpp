@@ -91,6 +91,8 @@ pf_status Server::start() {
     grid_ = step_grid(e_.backend, e_.n_cu);
     up_grid_ = e_.n_cu * 4;
     stop_ = false;
+    spin_stop_.store(false);
+    queued_.store(queue_.size());
     // the serving thread captures the step graph itself and then replays it,
     // so capture and every replay happen on one thread; start() returns the
     // capture's outcome
@@ -154,6 +156,17 @@ bool Server::wait_step(u32 seq) {
     return true;
 }
 
+// With no job running, the serving thread watches for a submission for up to
+// kIdleSpinS before it sleeps on the condition variable, so that a request
+// arriving meanwhile starts without the wake-up of a sleeping thread.
+void Server::idle_spin() {
+    constexpr double kIdleSpinS = 0.020;
+    const double t0 = now_s();
+    while (queued_.load(std::memory_order_acquire) == 0 && !spin_stop_.load(std::memory_order_acquire) &&
+           now_s() - t0 < kIdleSpinS)
+        __builtin_ia32_pause();
+}
+
 // Workgroups of the mixed step launch that start on the write tiles: a share
 // of the workgroups that get a tile (those past the step's tile count leave
 // at once) proportional to the write tiles' part of the step's work, a write
@@ -175,6 +188,7 @@ void Server::stop() {
             std::lock_guard<std::mutex> g(mu_);
             stop_ = true;
         }
+        spin_stop_.store(true, std::memory_order_release);
         cv_.notify_all();
         if (th_.joinable()) th_.join();
         std::lock_guard<std::mutex> g(mu_);
@@ -280,6 +294,7 @@ pf_status Server::submit(const pf_pos48& pos, int depth, u64 tag) {
         if (!running_ || stop_) return PF_EBUSY;
         if (queue_.size() > 1000000) return PF_EBUSY;
         queue_.push_back(std::move(j));
+        queued_.store(queue_.size(), std::memory_order_release);
     }
     cv_.notify_all();
     return PF_OK;
@@ -604,6 +619,7 @@ void Server::run() {
     std::vector<int> admitted;
     for (;;) {
         admitted.clear();
+        if (active.empty()) idle_spin();
         {
             std::unique_lock<std::mutex> g(mu_);
             cv_.wait(g, [&] { return stop_ || !queue_.empty() || !active.empty(); });
@@ -638,6 +654,7 @@ void Server::run() {
                 admitted.push_back(slot);
             }
             queue_idle_ = queue_.empty();
+            queued_.store(queue_.size(), std::memory_order_release);
         }
         StepTable& t = *h_tab_;
         t.nseg = t.nup = t.nroot = 0;

@@ -1,5 +1,6 @@
 // serve.h -- continuous-batching perft server.
 #pragma once
+#include <atomic>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -66,6 +67,7 @@ class Server {
     bool alloc_level(Job& j, u64 n, Engine::TLevel* lv);
     pf_status build_graph(int fine, int late, int rounds);
     bool wait_step(u32 seq);
+    void idle_spin();
     u32 mix_write_wgs(u32 write_tiles, u32 leaf_tiles) const;
 
     Engine& e_;
@@ -75,6 +77,10 @@ class Server {
     std::mutex mu_;
     std::condition_variable cv_;
     bool stop_ = false, running_ = false;
+    // read without the lock by the idle serving thread's spin: requests
+    // waiting for admission, and stop() having been called
+    std::atomic<size_t> queued_{0};
+    std::atomic<bool> spin_stop_{false};
     bool queue_idle_ = false;  // nothing was waiting for admission this step (server thread)
     int graph_status_ = 0;  // the serving thread's graph capture: 1 pending, else a pf_status
     std::deque<Job> queue_;        // submitted, not yet admitted
