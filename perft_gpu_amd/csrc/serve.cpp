@@ -139,18 +139,18 @@ pf_status Server::build_graph(int fine, int late, int rounds) {
     return PF_OK;
 }
 
-// Waits for step `seq`. The step's last kernel writes seq to the host table's
-// done word after every outcome has reached the host, a few microseconds
-// before the stream's completion would wake a waiting thread, so the serving
-// thread watches that word. A step still running after kSpinWaitS is waited
-// for by hipStreamSynchronize, which also reports a failed launch.
-bool Server::wait_step(u32 seq) {
+// Waits until the word *w of the host table holds v. The stream's last kernel
+// writes it after everything the host needs has reached the host, a few
+// microseconds before the stream's completion would wake a waiting thread, so
+// the serving thread watches the word (a step's done, a read-back's rb_done).
+// Work still running after kSpinWaitS is waited for by hipStreamSynchronize,
+// which also reports a failed launch.
+bool Server::wait_word(const u32* w, u32 v) {
     constexpr double kSpinWaitS = 0.001;
     const double t0 = now_s();
-    while (__atomic_load_n(&h_tab_->done, __ATOMIC_ACQUIRE) != seq) {
+    while (__atomic_load_n(w, __ATOMIC_ACQUIRE) != v) {
         if (now_s() - t0 > kSpinWaitS)
-            return hipStreamSynchronize(e_.stream) == hipSuccess &&
-                   __atomic_load_n(&h_tab_->done, __ATOMIC_ACQUIRE) == seq;
+            return hipStreamSynchronize(e_.stream) == hipSuccess && __atomic_load_n(w, __ATOMIC_ACQUIRE) == v;
         __builtin_ia32_pause();
     }
     return true;
@@ -209,6 +209,9 @@ void Server::stop() {
     hipFree(root_k2_);
     hipFree(root_p_);
     hipHostFree(h_tab_);
+    if (h_rb_) hipHostFree(h_rb_);
+    h_rb_ = nullptr;
+    rb_cap_ = 0;
     d_tab_ = nullptr; root_b_ = nullptr; root_w_ = nullptr; root_k1_ = nullptr; root_k2_ = nullptr;
     root_p_ = nullptr; slot_acc_ = nullptr; h_tab_ = nullptr; h_dev_ = nullptr;
 }
@@ -367,10 +370,24 @@ bool Server::chunk(Job& j, Frame& f, u64 budget, u64* cap) {
     if (f.lv.bsum && (f.off & (kBsumBlock - 1)) == 0) {
         const u64 nblk = (f.end + kBsumBlock - 1) >> kBsumShift;
         if (f.hbsum.size() != nblk) {
-            f.hbsum.assign((size_t)nblk, 0);
-            if (hipMemcpyAsync(f.hbsum.data(), f.lv.bsum, nblk * 4, hipMemcpyDeviceToHost, e_.stream) != hipSuccess ||
-                hipStreamSynchronize(e_.stream) != hipSuccess)
+            // through a pinned staging buffer, with a flag after the copy that
+            // the serving thread watches (no sleeping wait per read-back)
+            if (nblk > rb_cap_) {
+                if (h_rb_) hipHostFree(h_rb_);
+                h_rb_ = nullptr;
+                rb_cap_ = 0;
+                const u64 cap = std::max<u64>(nblk, 1u << 16);
+                if (hipHostMalloc(reinterpret_cast<void**>(&h_rb_), cap * 4, hipHostMallocDefault) != hipSuccess) {
+                    h_rb_ = nullptr;
+                    return false;
+                }
+                rb_cap_ = cap;
+            }
+            const u32 v = ++rb_seq_;
+            if (hipMemcpyAsync(h_rb_, f.lv.bsum, nblk * 4, hipMemcpyDeviceToHost, e_.stream) != hipSuccess ||
+                launch_post_flag(&h_dev_->rb_done, v, e_.stream) != hipSuccess || !wait_word(&h_tab_->rb_done, v))
                 return false;
+            f.hbsum.assign(h_rb_, h_rb_ + nblk);
         }
         u64 m = 0, sum = 0;
         for (u64 b = f.off >> kBsumShift; b < nblk; b++) {
@@ -776,7 +793,8 @@ void Server::run() {
         int rounds = 0;  // rounds are filled in order: round r + 1 only after round r
         while (rounds < kUpRounds && t.up_round[rounds + 1] > t.up_round[rounds]) rounds++;
         t.seq = ++seq_;
-        const bool ok = hipGraphLaunch(exec_[fine][t.nlate > 0][rounds], e_.stream) == hipSuccess && wait_step(t.seq);
+        const bool ok =
+            hipGraphLaunch(exec_[fine][t.nlate > 0][rounds], e_.stream) == hipSuccess && wait_word(&h_tab_->done, t.seq);
         const double t1 = now_s();
         {
             std::lock_guard<std::mutex> g(mu_);

@@ -66,7 +66,7 @@ class Server {
     char* take(Job& j, u64 bytes);
     bool alloc_level(Job& j, u64 n, Engine::TLevel* lv);
     pf_status build_graph(int fine, int late, int rounds);
-    bool wait_step(u32 seq);
+    bool wait_word(const u32* w, u32 v);
     void idle_spin();
     u32 mix_write_wgs(u32 write_tiles, u32 leaf_tiles) const;
 
@@ -107,6 +107,9 @@ class Server {
     std::vector<u32> pos_of_;      // a segment's position after the reordering
     bool late_ok_ = false;         // this step may hold late leaf passes
     u32 seq_ = 0;                  // steps launched
+    u32 rb_seq_ = 0;               // block-sum read-backs launched
+    u32* h_rb_ = nullptr;          // pinned staging of block-sum read-backs (rb_cap_ entries)
+    u64 rb_cap_ = 0;
     u32 late_src_[kMaxLate] = {};  // their write segments (indices before the reordering)
     // by tiling (1: fine), late leaf launch (1: recorded), then upsweep rounds
     hipGraph_t graph_[2][2][kUpRounds + 1] = {};

@@ -900,6 +900,15 @@ __global__ __launch_bounds__(256) void k_post_step(const StepTable* d, StepTable
         __hip_atomic_store(&h->done, d->seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+__global__ void k_post_flag(u32* flag, u32 v) {
+    if (threadIdx.x == 0) __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+hipError_t launch_post_flag(u32* flag, u32 v, hipStream_t s) {
+    hipLaunchKernelGGL(k_post_flag, dim3(1), dim3(64), 0, s, flag, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_fetch_step(StepTable* d, const StepTable* h, hipStream_t s) {
     hipLaunchKernelGGL(k_fetch_step, dim3(32), dim3(256), 0, s, d, h);
     return hipGetLastError();

@@ -169,7 +169,8 @@ struct StepTable {
     RootOut rout[kMaxRoots];
     unsigned long long acc[kMaxRoots];  // serving: each job slot's root accumulator
     unsigned long long stats[2];        // serving: table probes and hits of the write segments
-    u32 done, pad;                      // serving: seq of the last step whose outcomes are all in the host table
+    u32 done;                           // serving: seq of the last step whose outcomes are all in the host table
+    u32 rb_done;                        // serving: number of the last read-back that has reached the host
 };
 
 // ---- device-planned table pass ----------------------------------------------
@@ -237,6 +238,8 @@ hipError_t capture_plan_pass(int backend, const SliderTables* tabs, const ZTable
 // once every outcome is visible to the host.
 hipError_t launch_fetch_step(StepTable* d, const StepTable* h, hipStream_t s);
 hipError_t launch_post_step(const StepTable* d, StepTable* h, u32 max_jobs, hipStream_t s);
+// *flag = v (host memory, system scope) once the stream's earlier work is done.
+hipError_t launch_post_flag(u32* flag, u32 v, hipStream_t s);
 int step_grid(int backend, int n_cu);       // workgroups of the serving step's mixed launch
 int step_max_bpc(int backend, int leaf);  // the planned pass's write (0) / leaf (1) step kernel
 // fine: the mixed launch tiles by StepTable::mix_tp parents, not kTTTileParents;
