@@ -845,9 +845,9 @@ __global__ __launch_bounds__(256) void k_up_step(const StepTable* __restrict__ t
 
 // Serving: the step table's host part, read by the GPU from coherent host
 // memory -- only what the step uses (header, tile prefix, upsweep rounds and
-// the used segments, upsweeps and roots), so that a small step moves a few
-// hundred bytes rather than the whole table, and no copy node sits between
-// the host and the step's first kernel.  Copied as 4-byte words.
+// the used segments, upsweeps and roots), so that a small step moves about a
+// kilobyte rather than the whole table, and no copy node sits between the
+// host and the step's first kernel.  Copied as 4-byte words.
 __global__ __launch_bounds__(256) void k_fetch_step(StepTable* __restrict__ d, const StepTable* __restrict__ h) {
     __shared__ u32 hdr[3];
     if (threadIdx.x == 0) {
